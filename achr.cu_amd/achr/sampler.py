@@ -1,0 +1,244 @@
+"""Python view of the C ABI: models, null spaces, samplers.
+
+Mirrors the reference's host flow (ACHRcu/ACHRCuda.cu:504-802): read the
+model, compute the null space, read the warmup points and their centre, then
+run files of P chains x S steps. Arrays cross the boundary as numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import AchrOptions, check, lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Model:
+    """Stoichiometric model {x : S x = 0, lb <= x <= ub} held by libachr."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    @classmethod
+    def from_coo(cls, n: int, m: int, rows, cols, vals, lb, ub) -> "Model":
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        lb = np.ascontiguousarray(lb, dtype=np.float64)
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        if lb.shape != (n,) or ub.shape != (n,):
+            raise ValueError("bounds must have n entries")
+        h = C.c_void_p()
+        check(lib().achr_model_create(n, m, len(vals), _ip(rows), _ip(cols), _dp(vals), _dp(lb), _dp(ub),
+                                      C.byref(h)), "achr_model_create")
+        return cls(h)
+
+    @classmethod
+    def from_dense(cls, S: np.ndarray, lb, ub) -> "Model":
+        S = np.asarray(S, dtype=np.float64)
+        r, c = np.nonzero(S)
+        return cls.from_coo(S.shape[1], S.shape[0], r, c, S[r, c], lb, ub)
+
+    @classmethod
+    def read_mps(cls, path: str) -> "Model":
+        h = C.c_void_p()
+        check(lib().achr_model_read_mps(str(path).encode(), C.byref(h)), "achr_model_read_mps")
+        return cls(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().achr_model_destroy(h)
+            self._h = None
+
+    @property
+    def dims(self):
+        n, m, nnz, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().achr_model_dims(self._h, C.byref(n), C.byref(m), C.byref(nnz), C.byref(d)), "dims")
+        return n.value, m.value, nnz.value, d.value
+
+    @property
+    def n(self) -> int:
+        return self.dims[0]
+
+    @property
+    def m(self) -> int:
+        return self.dims[1]
+
+    def bounds(self):
+        n = self.n
+        lb = np.empty(n)
+        ub = np.empty(n)
+        check(lib().achr_model_bounds(self._h, _dp(lb), _dp(ub)), "bounds")
+        return lb, ub
+
+    def csr(self):
+        n, m, nnz, _ = self.dims
+        rp = np.empty(m + 1, dtype=np.int32)
+        ci = np.empty(max(nnz, 1), dtype=np.int32)
+        va = np.empty(max(nnz, 1))
+        check(lib().achr_model_csr(self._h, _ip(rp), _ip(ci), _dp(va)), "csr")
+        return rp, ci[:nnz], va[:nnz]
+
+    def dense(self) -> np.ndarray:
+        n, m, _, _ = self.dims
+        rp, ci, va = self.csr()
+        S = np.zeros((m, n))
+        for r in range(m):
+            S[r, ci[rp[r]:rp[r + 1]]] = va[rp[r]:rp[r + 1]]
+        return S
+
+    def compute_nullspace(self) -> int:
+        d = C.c_int()
+        check(lib().achr_model_compute_nullspace(self._h, C.byref(d)), "achr_model_compute_nullspace")
+        return d.value
+
+    def set_nullspace(self, N: np.ndarray) -> None:
+        N = np.asfortranarray(N, dtype=np.float64)
+        n, d = N.shape
+        if n != self.n:
+            raise ValueError("N must have n rows")
+        flat = np.ascontiguousarray(N.ravel(order="F"))
+        check(lib().achr_model_set_nullspace(self._h, _dp(flat), d), "achr_model_set_nullspace")
+
+    def nullspace(self) -> np.ndarray:
+        n, _, _, d = self.dims
+        flat = np.empty(max(n * d, 1))
+        check(lib().achr_model_get_nullspace(self._h, _dp(flat)), "achr_model_get_nullspace")
+        return flat[: n * d].reshape((n, d), order="F")
+
+
+@dataclass
+class Options:
+    """achr_options with the reference defaults (ACHRCuda.cu:505-512, :325, :338)."""
+
+    dTol: float = 1e-14
+    uTol: float = 1e-9
+    maxMinTol: float = 1e-9
+    reprojTol: float = 1e-9
+    check_every: int = 10
+    max_misses: int = 100000
+    lanes: int = 0
+    reproject_mode: int = 0
+    trace_chains: int = 0
+    trace_thin: int = 0
+
+    def to_c(self) -> AchrOptions:
+        return AchrOptions(self.dTol, self.uTol, self.maxMinTol, self.reprojTol, self.check_every,
+                           self.max_misses, self.lanes, self.reproject_mode, self.trace_chains, self.trace_thin)
+
+
+class Sampler:
+    """Device-resident sampler over a model, its null space and warmup set."""
+
+    def __init__(self, model: Model, warmup: np.ndarray | None = None, opts: Options | None = None,
+                 warmup_csv: str | None = None):
+        self.model = model
+        self.opts = opts or Options()
+        co = self.opts.to_c()
+        h = C.c_void_p()
+        if warmup_csv is not None:
+            check(lib().achr_sampler_create_csv(model._h, str(warmup_csv).encode(), C.byref(co), C.byref(h)),
+                  "achr_sampler_create_csv")
+        else:
+            w = np.asfortranarray(warmup, dtype=np.float64)
+            if w.shape[0] != model.n:
+                raise ValueError("warmup must be n x W")
+            flat = np.ascontiguousarray(w.ravel(order="F"))
+            check(lib().achr_sampler_create(model._h, _dp(flat), w.shape[1], C.byref(co), C.byref(h)),
+                  "achr_sampler_create")
+        self._h = h
+        self.n = model.n
+        self.P = 0
+        self.S = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().achr_sampler_destroy(h)
+            self._h = None
+
+    def center(self) -> np.ndarray:
+        c = np.empty(self.n)
+        check(lib().achr_sampler_center(self._h, _dp(c)), "center")
+        return c
+
+    def schedule(self):
+        lanes, mode = C.c_int(), C.c_int()
+        check(lib().achr_sampler_schedule(self._h, C.byref(lanes), C.byref(mode)), "schedule")
+        return lanes.value, mode.value
+
+    def set_stream(self, stream_ptr: int) -> None:
+        check(lib().achr_sampler_set_stream(self._h, C.c_void_p(stream_ptr)), "set_stream")
+
+    def run(self, P: int, S: int, seed: int = 0, file: int = 0, chain0: int = 0) -> None:
+        """Queues one file (P chains x S accepted steps); results stay on device."""
+        check(lib().achr_sample(self._h, P, S, C.c_uint64(seed), C.c_uint32(file), chain0), "achr_sample")
+        self.P, self.S = P, S
+
+    def points(self) -> np.ndarray:
+        """n x P, reaction major (the reference's File<k> layout)."""
+        out = np.empty((self.n, self.P))
+        check(lib().achr_sample_points(self._h, _dp(out)), "achr_sample_points")
+        return out
+
+    def stats(self) -> np.ndarray:
+        st = np.empty((self.P, 4), dtype=np.int32)
+        check(lib().achr_sample_stats(self._h, st.ctypes.data_as(C.POINTER(C.c_int32))), "achr_sample_stats")
+        return st
+
+    def trace(self) -> np.ndarray:
+        tc = min(self.opts.trace_chains, self.P)
+        L = self.S // self.opts.trace_thin
+        tr = np.empty((tc, L, self.n))
+        check(lib().achr_sample_trace(self._h, _dp(tr)), "achr_sample_trace")
+        return tr
+
+    def timing(self):
+        tot, stp, prj = C.c_double(), C.c_double(), C.c_double()
+        ns, np_ = C.c_int(), C.c_int()
+        check(lib().achr_sample_timing(self._h, C.byref(tot), C.byref(stp), C.byref(prj), C.byref(ns),
+                                       C.byref(np_)), "achr_sample_timing")
+        return {"ms_total": tot.value, "ms_step": stp.value, "ms_proj": prj.value,
+                "step_launches": ns.value, "proj_launches": np_.value}
+
+    def sample(self, P: int, S: int, seed: int = 0, file: int = 0, chain0: int = 0) -> np.ndarray:
+        self.run(P, S, seed, file, chain0)
+        return self.points()
+
+
+def philox4x32_10(ctr, key) -> tuple:
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().achr_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def write_points_csv(path: str, points: np.ndarray) -> None:
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    check(lib().achr_write_points_csv(str(path).encode(), _dp(p), p.shape[0], p.shape[1]), "write_points_csv")
+
+
+def read_warmup_csv(path: str, n_expected: int = 0) -> np.ndarray:
+    ptr = C.POINTER(C.c_double)()
+    W = C.c_int()
+    check(lib().achr_read_warmup_csv(str(path).encode(), n_expected, C.byref(ptr), C.byref(W)),
+          "achr_read_warmup_csv")
+    try:
+        # column major n x W -> numpy (n, W)
+        flat = np.ctypeslib.as_array(ptr, shape=(W.value * max(n_expected, 1),)) if n_expected else None
+        if flat is None:
+            raise ValueError("n_expected is required")
+        return flat.reshape((n_expected, W.value), order="F").copy()
+    finally:
+        lib().achr_free(ptr)

@@ -1,0 +1,221 @@
+"""Seeded synthetic flux models and warmup sets.
+
+The reference ships one real model (P. putida, VFWarmup/lib/P_Putida.mps) and
+no sampler inputs; its warmup sets come from VFWarmup, an LP (GLPK/CPLEX +
+MPI) program. The workloads of this project are synthetic models with the
+structure of metabolic networks:
+
+* S is m x n, each column with 2-4 nonzeros from {+-1, +-2} on distinct rows
+  (P(2, 3, 4) = (.3, .2, .5): 3.2 per column on average);
+* an interior point x0 is drawn in null(S) (|x0|_inf <= x0_scale);
+* bounds are x0 -+ U(lo, hi); a fraction of reactions is irreversible
+  (lb raised to 0 where x0 > 0) and a few are fixed (lb = ub = x0);
+* bounds are rounded to 4 decimals so the MPS file carries them exactly in
+  12-character fixed-format fields.
+
+Warmup sets:
+* ``fva_warmup``: the reference generator's recipe (createWarmupPtsGLPK.c):
+  for reaction i, minimise then maximise x_i (columns 2i and 2i+1, :101-173),
+  clip to bounds (movePtsBds, :36-46), then move every point to
+  0.33 x + 0.67 c where c is their mean (:48-65, :283-284). Solved with
+  scipy's HiGHS here (GLPK is not available); only for small models.
+* ``chord_warmup``: for models too large for 2n LPs: endpoints of random
+  chords through x0 along null-space directions (alternately the far and near
+  end, like the max / min pair of an FVA), then the same 0.33 / 0.67 move.
+* ``round_like_csv``: the "%f" text the reference's warmup writer produces
+  (createWarmupPtsGLPK.c:286-297) keeps 6 decimals; sampling from such a file
+  sees residuals |S w| ~ 1e-6, which is what triggers the reference's
+  reprojection every 10 steps.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class SynthModel:
+    S: np.ndarray | None          # dense m x n (small models) or None
+    rows: np.ndarray              # COO of S
+    cols: np.ndarray
+    vals: np.ndarray
+    m: int
+    n: int
+    lb: np.ndarray
+    ub: np.ndarray
+    x0: np.ndarray
+    N: np.ndarray | None = None   # orthonormal null-space basis used to draw x0
+
+    def dense(self) -> np.ndarray:
+        if self.S is not None:
+            return self.S
+        S = np.zeros((self.m, self.n))
+        np.add.at(S, (self.rows, self.cols), self.vals)
+        return S
+
+
+def random_stoich(m: int, n: int, rng: np.random.Generator, probs=(0.3, 0.2, 0.5)):
+    """COO triplets of an m x n matrix with 2-4 nonzeros per column in {+-1, +-2}."""
+    k = rng.choice([2, 3, 4], size=n, p=probs)
+    rows, cols, vals = [], [], []
+    for j in range(n):
+        r = rng.choice(m, size=int(k[j]), replace=False)
+        v = rng.choice([-2.0, -1.0, 1.0, 2.0], size=int(k[j]))
+        rows.append(r)
+        cols.append(np.full(k[j], j))
+        vals.append(v)
+    return (np.concatenate(rows).astype(np.int32), np.concatenate(cols).astype(np.int32),
+            np.concatenate(vals))
+
+
+def nullspace_dense(S: np.ndarray) -> np.ndarray:
+    """Orthonormal basis of null(S) by SVD (small models)."""
+    import scipy.linalg
+    return scipy.linalg.null_space(S)
+
+
+def make_model(m: int, n: int, seed: int, *, irrev_frac: float, n_fixed: int, width=(5.0, 50.0),
+               x0_scale: float = 10.0, N: np.ndarray | None = None, nullspace_fn=None) -> SynthModel:
+    """Synthetic model; N (or nullspace_fn(rows, cols, vals, m, n)) supplies null(S) for large n."""
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = random_stoich(m, n, rng)
+    S = None
+    if N is None:
+        if nullspace_fn is not None:
+            N = nullspace_fn(rows, cols, vals, m, n)
+        else:
+            S = np.zeros((m, n))
+            np.add.at(S, (rows, cols), vals)
+            N = nullspace_dense(S)
+    d = N.shape[1]
+    if d == 0:
+        raise ValueError("S has a trivial null space")
+    z = rng.standard_normal(d)
+    x0 = N @ z
+    x0 *= x0_scale * rng.uniform(0.5, 1.0) / np.max(np.abs(x0))
+    fixed = rng.choice(n, size=n_fixed, replace=False) if n_fixed else np.array([], dtype=int)
+    if len(fixed):
+        # Move x0 inside null(S) so that the fixed reactions sit exactly on
+        # their 4-decimal values.
+        target = np.round(x0[fixed], 4)
+        NF = N[fixed, :]
+        delta, *_ = np.linalg.lstsq(NF, target - x0[fixed], rcond=None)
+        x0 = x0 + N @ delta
+    lo, hi = width
+    lb = np.round(x0 - rng.uniform(lo, hi, size=n), 4)
+    ub = np.round(x0 + rng.uniform(lo, hi, size=n), 4)
+    irrev = rng.random(n) < irrev_frac
+    lb = np.where(irrev & (x0 > 0), np.maximum(lb, 0.0), lb)
+    if len(fixed):
+        v = np.round(x0[fixed], 4)
+        lb[fixed] = v
+        ub[fixed] = v
+    if S is None and n <= 4000:
+        S = np.zeros((m, n))
+        np.add.at(S, (rows, cols), vals)
+    return SynthModel(S, rows, cols, vals, m, n, lb, ub, x0, N)
+
+
+def config1(seed: int = 1) -> SynthModel:
+    """E. coli-core-sized model: 72 x 95, 30% irreversible, 5 fixed."""
+    return make_model(72, 95, seed, irrev_frac=0.30, n_fixed=5, width=(5.0, 50.0), x0_scale=10.0)
+
+
+def config2(nullspace_fn, seed: int = 2) -> SynthModel:
+    """Genome-scale model: 5,800 x 10,600, 40% irreversible, 2% fixed, bounds +-(100..1000)."""
+    return make_model(5800, 10600, seed, irrev_frac=0.40, n_fixed=212, width=(100.0, 1000.0), x0_scale=100.0,
+                      nullspace_fn=nullspace_fn)
+
+
+def move_to_center(W: np.ndarray) -> np.ndarray:
+    """0.33 x + 0.67 c for every column (createWarmupPtsGLPK.c:59-65)."""
+    c = W.sum(axis=1) / W.shape[1]
+    return W * 0.33 + 0.67 * c[:, None]
+
+
+def fva_warmup(model: SynthModel) -> np.ndarray:
+    """2n FVA points (min, max of each reaction), clipped and moved to the centre."""
+    from scipy.optimize import linprog
+    S = model.dense()
+    n = model.n
+    W = np.empty((n, 2 * n))
+    bounds = list(zip(model.lb, model.ub))
+    for i in range(n):
+        for sense, col in ((1.0, 2 * i), (-1.0, 2 * i + 1)):  # min -> 2i, max -> 2i+1
+            c = np.zeros(n)
+            c[i] = sense
+            res = linprog(c, A_eq=S, b_eq=np.zeros(model.m), bounds=bounds, method="highs")
+            if res.status != 0:
+                raise RuntimeError(f"LP {i} ({'min' if sense > 0 else 'max'}) failed: {res.message}")
+            W[:, col] = np.clip(res.x, model.lb, model.ub)
+    return move_to_center(W)
+
+
+def chord_warmup(N: np.ndarray, x0: np.ndarray, lb: np.ndarray, ub: np.ndarray, W: int, seed: int,
+                 shrink: float = 0.999) -> np.ndarray:
+    """W points on random null-space chords through x0, then moved to the centre."""
+    rng = np.random.default_rng(seed)
+    n, d = N.shape
+    out = np.empty((n, W))
+    B = 512
+    for k0 in range(0, W, B):
+        k1 = min(W, k0 + B)
+        G = rng.standard_normal((d, k1 - k0))
+        V = N @ G
+        with np.errstate(divide="ignore", invalid="ignore"):
+            up = np.where(V > 1e-12, (ub - x0)[:, None] / V, np.where(V < -1e-12, (lb - x0)[:, None] / V, np.inf))
+            dn = np.where(V > 1e-12, (lb - x0)[:, None] / V, np.where(V < -1e-12, (ub - x0)[:, None] / V, -np.inf))
+        tmax = up.min(axis=0)
+        tmin = dn.max(axis=0)
+        far = (np.arange(k0, k1) % 2) == 1
+        t = np.where(far, tmax, tmin) * shrink
+        out[:, k0:k1] = x0[:, None] + V * t[None, :]
+    out = np.clip(out, lb[:, None], ub[:, None])
+    return move_to_center(out)
+
+
+def round_like_csv(W: np.ndarray) -> np.ndarray:
+    """Values as a "%f" warmup CSV stores them (6 decimals)."""
+    return np.round(W, 6)
+
+
+def fmt_g12(v: float) -> str:
+    s = f"{v:.12g}"
+    if len(s) > 12:
+        s = f"{v:.6f}".rstrip("0").rstrip(".")
+    return s
+
+
+def write_mps(model: SynthModel, path: str, name: str = "synthetic") -> None:
+    """Fixed-format MPS in the dialect of VFWarmup/test_fva_validation.py:58-86."""
+    order = np.lexsort((model.rows, model.cols))
+    rows, cols, vals = model.rows[order], model.cols[order], model.vals[order]
+    with open(path, "w") as f:
+        f.write(f"{'NAME':<14}{name}\n")
+        f.write("ROWS\n")
+        f.write(" N  COST\n")
+        for i in range(model.m):
+            f.write(f" E  EQ{i + 1}\n")
+        f.write("COLUMNS\n")
+        for r, c, v in zip(rows, cols, vals):
+            f.write(f"    {'X' + str(c + 1):<10}{'EQ' + str(r + 1):<10}{fmt_g12(v):<12}\n")
+        f.write("RHS\n")
+        f.write("BOUNDS\n")
+        for j in range(model.n):
+            cn = f"X{j + 1}"
+            lb, ub = model.lb[j], model.ub[j]
+            if lb == ub:
+                f.write(f" FX BOUND     {cn:<10}{fmt_g12(lb):<12}\n")
+            else:
+                if lb != 0:
+                    f.write(f" LO BOUND     {cn:<10}{fmt_g12(lb):<12}\n")
+                f.write(f" UP BOUND     {cn:<10}{fmt_g12(ub):<12}\n")
+        f.write("ENDATA\n")
+
+
+def write_warmup_csv(W: np.ndarray, path: str, fmt: str = "%f") -> None:
+    """One line per reaction, one comma-separated column per point (createWarmupPtsGLPK.c:286-297)."""
+    with open(path, "w") as f:
+        for i in range(W.shape[0]):
+            f.write(",".join(fmt % v for v in W[i]) + "\n")

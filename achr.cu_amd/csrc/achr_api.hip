@@ -1,0 +1,816 @@
+// C ABI of the ACHR sampler: models, null space, warmup centre, sampling runs,
+// file formats. Host code; the device work is in achr_step.hip and
+// achr_project.hip.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/achr.h"
+#include "achr_internal.h"
+#include "achr_philox.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(ACHR_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= count && p) return hipSuccess;
+    release();
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+    if (e == hipSuccess) count = n;
+    return e;
+  }
+  hipError_t upload(const T* h, size_t n, hipStream_t s) {
+    hipError_t e = ensure(n);
+    if (e != hipSuccess || n == 0) return e;
+    e = hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+};
+
+}  // namespace
+
+struct achr_model {
+  int n = 0, m = 0;
+  std::vector<int> rowptr, colidx;
+  std::vector<double> vals, lb, ub;
+  std::vector<double> N;  // n x d column major
+  int d = -1;             // -1: not computed yet
+};
+
+struct achr_sampler {
+  int n = 0, m = 0, W = 0, d = 0;
+  achr_options opt{};
+  achr::Schedule sched{};
+  int mode = 1;
+  hipStream_t stream = nullptr;
+  std::vector<double> c0;
+  DevBuf<double> warm, lb, ub, c0d, vals, Ncm, Nrm;
+  DevBuf<int> rowptr, colidx;
+  // run state
+  int P = 0, S = 0;
+  DevBuf<double> X, C, T, out, trace;
+  DevBuf<int4> st, stats;
+  DevBuf<int> flag, idx, count;
+  int trace_len = 0;
+  // timing
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_step, ev_proj;  // indices into ev_pool
+  bool have_run = false;
+};
+
+extern "C" {
+
+const char* achr_last_error(void) { return g_err.c_str(); }
+const char* achr_version(void) { return "achr-mi355x 0.1.0 (gfx950)"; }
+
+void achr_free(void* p) { std::free(p); }
+
+void achr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  const achr::u32x4 r = achr::philox4x32_10({ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+  out[0] = r.v0;
+  out[1] = r.v1;
+  out[2] = r.v2;
+  out[3] = r.v3;
+}
+
+void achr_options_default(achr_options* o) {
+  if (!o) return;
+  o->dTol = 1e-14;
+  o->uTol = 1e-9;
+  o->maxMinTol = 1e-9;
+  o->reprojTol = 1e-9;
+  o->check_every = 10;
+  o->max_misses = 100000;
+  o->lanes = 0;
+  o->reproject_mode = 0;
+  o->trace_chains = 0;
+  o->trace_thin = 0;
+}
+
+// ------------------------------------------------------------------ model
+
+static int build_model(int n, int m, int nnz, const int* rows, const int* cols, const double* vals,
+                       const double* lb, const double* ub, achr_model** out) {
+  if (!out) return fail(ACHR_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (n <= 0 || m < 0 || nnz < 0) return fail(ACHR_E_ARG, "bad sizes n=%d m=%d nnz=%d", n, m, nnz);
+  if (nnz > 0 && (!rows || !cols || !vals)) return fail(ACHR_E_ARG, "NULL triplet arrays");
+  if (!lb || !ub) return fail(ACHR_E_ARG, "NULL bounds");
+  std::vector<std::map<int, double>> rowmap(static_cast<size_t>(m));
+  for (int k = 0; k < nnz; ++k) {
+    if (rows[k] < 0 || rows[k] >= m || cols[k] < 0 || cols[k] >= n)
+      return fail(ACHR_E_ARG, "entry %d out of range (row %d col %d)", k, rows[k], cols[k]);
+    rowmap[rows[k]][cols[k]] += vals[k];
+  }
+  auto* M = new (std::nothrow) achr_model();
+  if (!M) return fail(ACHR_E_NOMEM, "model allocation");
+  M->n = n;
+  M->m = m;
+  M->rowptr.assign(static_cast<size_t>(m) + 1, 0);
+  for (int i = 0; i < m; ++i) {
+    for (const auto& kv : rowmap[i]) {
+      if (kv.second == 0.0) continue;
+      M->colidx.push_back(kv.first);
+      M->vals.push_back(kv.second);
+    }
+    M->rowptr[i + 1] = static_cast<int>(M->colidx.size());
+  }
+  M->lb.assign(lb, lb + n);
+  M->ub.assign(ub, ub + n);
+  *out = M;
+  return ACHR_OK;
+}
+
+int achr_model_create(int n_rxns, int n_mets, int nnz, const int* rows, const int* cols, const double* vals,
+                      const double* lb, const double* ub, achr_model** out) {
+  return build_model(n_rxns, n_mets, nnz, rows, cols, vals, lb, ub, out);
+}
+
+void achr_model_destroy(achr_model* model) { delete model; }
+
+int achr_model_dims(const achr_model* M, int* n, int* m, int* nnz, int* d) {
+  if (!M) return fail(ACHR_E_ARG, "model is NULL");
+  if (n) *n = M->n;
+  if (m) *m = M->m;
+  if (nnz) *nnz = static_cast<int>(M->vals.size());
+  if (d) *d = M->d;
+  return ACHR_OK;
+}
+
+int achr_model_bounds(const achr_model* M, double* lb, double* ub) {
+  if (!M) return fail(ACHR_E_ARG, "model is NULL");
+  if (lb) std::copy(M->lb.begin(), M->lb.end(), lb);
+  if (ub) std::copy(M->ub.begin(), M->ub.end(), ub);
+  return ACHR_OK;
+}
+
+int achr_model_csr(const achr_model* M, int* rowptr, int* colidx, double* vals) {
+  if (!M) return fail(ACHR_E_ARG, "model is NULL");
+  if (rowptr) std::copy(M->rowptr.begin(), M->rowptr.end(), rowptr);
+  if (colidx) std::copy(M->colidx.begin(), M->colidx.end(), colidx);
+  if (vals) std::copy(M->vals.begin(), M->vals.end(), vals);
+  return ACHR_OK;
+}
+
+// -------------------------------------------------------------- MPS reader
+
+static bool next_tokens(const char* line, std::vector<std::string>& tok) {
+  tok.clear();
+  const char* p = line;
+  while (*p) {
+    while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n') ++p;
+    if (!*p) break;
+    const char* q = p;
+    while (*q && *q != ' ' && *q != '\t' && *q != '\r' && *q != '\n') ++q;
+    tok.emplace_back(p, q);
+    p = q;
+  }
+  return !tok.empty();
+}
+
+static bool parse_num(const std::string& s, double* v) {
+  errno = 0;
+  char* end = nullptr;
+  *v = std::strtod(s.c_str(), &end);
+  return end && *end == '\0' && errno != ERANGE;
+}
+
+int achr_model_read_mps(const char* path, achr_model** out) {
+  if (!path || !out) return fail(ACHR_E_ARG, "NULL argument");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(ACHR_E_IO, "cannot open MPS file %s", path);
+  enum Sec { NONE, ROWS, COLUMNS, RHS, RANGES, BOUNDS, DONE } sec = NONE;
+  std::map<std::string, int> row_index;  // constraint rows only
+  std::string obj_row;
+  std::map<std::string, int> col_index;
+  std::vector<int> rr, cc;
+  std::vector<double> vv;
+  std::vector<double> lb, ub;
+  std::vector<std::string> tok;
+  std::vector<char> buf(1 << 16);
+  int lineno = 0;
+  auto col_of = [&](const std::string& name) {
+    auto it = col_index.find(name);
+    if (it != col_index.end()) return it->second;
+    const int j = static_cast<int>(col_index.size());
+    col_index.emplace(name, j);
+    lb.push_back(0.0);
+    ub.push_back(DBL_MAX);
+    return j;
+  };
+  int rc = ACHR_OK;
+  while (std::fgets(buf.data(), static_cast<int>(buf.size()), f)) {
+    ++lineno;
+    if (buf[0] == '*') continue;
+    const bool header = buf[0] != ' ' && buf[0] != '\t' && buf[0] != '\n' && buf[0] != '\r';
+    if (!next_tokens(buf.data(), tok)) continue;
+    if (header) {
+      const std::string& h = tok[0];
+      if (h == "NAME") sec = NONE;
+      else if (h == "ROWS") sec = ROWS;
+      else if (h == "COLUMNS") sec = COLUMNS;
+      else if (h == "RHS") sec = RHS;
+      else if (h == "RANGES") sec = RANGES;
+      else if (h == "BOUNDS") sec = BOUNDS;
+      else if (h == "ENDATA") { sec = DONE; break; }
+      else if (h == "OBJSENSE" || h == "OBJSENS") sec = NONE;
+      else { rc = fail(ACHR_E_FORMAT, "%s:%d: unknown section %s", path, lineno, h.c_str()); break; }
+      continue;
+    }
+    if (sec == ROWS) {
+      if (tok.size() < 2) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad ROWS line", path, lineno); break; }
+      const std::string& type = tok[0];
+      if (type == "N" && obj_row.empty()) { obj_row = tok[1]; continue; }
+      if (type != "N" && type != "E" && type != "L" && type != "G") {
+        rc = fail(ACHR_E_FORMAT, "%s:%d: bad row type %s", path, lineno, type.c_str());
+        break;
+      }
+      const int i = static_cast<int>(row_index.size());
+      row_index.emplace(tok[1], i);
+    } else if (sec == COLUMNS) {
+      if (tok.size() >= 3 && (tok[1] == "'MARKER'" || tok[2] == "'MARKER'")) continue;
+      if (tok.size() != 3 && tok.size() != 5) {
+        rc = fail(ACHR_E_FORMAT, "%s:%d: bad COLUMNS line", path, lineno);
+        break;
+      }
+      const int j = col_of(tok[0]);
+      for (size_t k = 1; k + 1 < tok.size(); k += 2) {
+        double v;
+        if (!parse_num(tok[k + 1], &v)) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad number", path, lineno); break; }
+        if (tok[k] == obj_row) continue;
+        auto it = row_index.find(tok[k]);
+        if (it == row_index.end()) {
+          rc = fail(ACHR_E_FORMAT, "%s:%d: unknown row %s", path, lineno, tok[k].c_str());
+          break;
+        }
+        rr.push_back(it->second);
+        cc.push_back(j);
+        vv.push_back(v);
+      }
+      if (rc != ACHR_OK) break;
+    } else if (sec == BOUNDS) {
+      if (tok.size() < 3) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad BOUNDS line", path, lineno); break; }
+      const std::string& type = tok[0];
+      const bool novalue = type == "FR" || type == "MI" || type == "PL" || type == "BV";
+      // Bound name may be omitted when the line has one field fewer.
+      std::string cname;
+      double v = 0.0;
+      if (novalue) {
+        cname = tok.size() >= 3 ? tok[2] : tok[1];
+      } else {
+        if (tok.size() < 4) { rc = fail(ACHR_E_FORMAT, "%s:%d: missing bound value", path, lineno); break; }
+        cname = tok[2];
+        if (!parse_num(tok[3], &v)) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad number", path, lineno); break; }
+      }
+      auto it = col_index.find(cname);
+      if (it == col_index.end()) {
+        rc = fail(ACHR_E_FORMAT, "%s:%d: unknown column %s", path, lineno, cname.c_str());
+        break;
+      }
+      const int j = it->second;
+      if (type == "UP") {
+        ub[j] = v;
+        if (v < 0.0 && lb[j] == 0.0) lb[j] = -DBL_MAX;  // MPS convention for a negative UP bound
+      } else if (type == "LO") {
+        lb[j] = v;
+      } else if (type == "FX") {
+        lb[j] = v;
+        ub[j] = v;
+      } else if (type == "FR") {
+        lb[j] = -DBL_MAX;
+        ub[j] = DBL_MAX;
+      } else if (type == "MI") {
+        lb[j] = -DBL_MAX;
+      } else if (type == "PL") {
+        ub[j] = DBL_MAX;
+      } else if (type == "BV") {
+        lb[j] = 0.0;
+        ub[j] = 1.0;
+      } else if (type == "LI") {
+        lb[j] = v;
+      } else if (type == "UI") {
+        ub[j] = v;
+      } else {
+        rc = fail(ACHR_E_FORMAT, "%s:%d: bad bound type %s", path, lineno, type.c_str());
+        break;
+      }
+    }
+    // RHS and RANGES are ignored, as the reference ignores them (it samples S x = 0).
+  }
+  std::fclose(f);
+  if (rc != ACHR_OK) return rc;
+  if (sec != DONE) return fail(ACHR_E_FORMAT, "%s: missing ENDATA", path);
+  const int n = static_cast<int>(col_index.size());
+  const int m = static_cast<int>(row_index.size());
+  if (n == 0) return fail(ACHR_E_FORMAT, "%s: no columns", path);
+  return build_model(n, m, static_cast<int>(vv.size()), rr.data(), cc.data(), vv.data(), lb.data(), ub.data(),
+                     out);
+}
+
+// -------------------------------------------------------------- null space
+
+int achr_model_set_nullspace(achr_model* M, const double* N, int d) {
+  if (!M) return fail(ACHR_E_ARG, "model is NULL");
+  if (d < 0 || d > M->n || (d > 0 && !N)) return fail(ACHR_E_ARG, "bad null-space dimension %d", d);
+  M->N.assign(N, N + static_cast<size_t>(M->n) * d);
+  M->d = d;
+  return ACHR_OK;
+}
+
+int achr_model_get_nullspace(const achr_model* M, double* N) {
+  if (!M) return fail(ACHR_E_ARG, "model is NULL");
+  if (M->d < 0) return fail(ACHR_E_STATE, "null space not computed");
+  if (N) std::copy(M->N.begin(), M->N.end(), N);
+  return ACHR_OK;
+}
+
+int achr_model_compute_nullspace(achr_model* M, int* null_dim) {
+  if (!M) return fail(ACHR_E_ARG, "model is NULL");
+  const int n = M->n;
+  // S^T S from the CSR rows on the host: (S^T S)_{ab} = sum_r S_ra S_rb.
+  std::vector<double> A(static_cast<size_t>(n) * n, 0.0);
+  for (int r = 0; r < M->m; ++r) {
+    for (int p = M->rowptr[r]; p < M->rowptr[r + 1]; ++p)
+      for (int q = M->rowptr[r]; q < M->rowptr[r + 1]; ++q)
+        A[static_cast<size_t>(M->colidx[p]) + static_cast<size_t>(M->colidx[q]) * n] += M->vals[p] * M->vals[q];
+  }
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return fail(ACHR_E_SOLVER, "rocblas_create_handle");
+  double *dA = nullptr, *dD = nullptr, *dE = nullptr;
+  rocblas_int* dinfo = nullptr;
+  int rc = ACHR_OK;
+  std::vector<double> ev(n);
+  rocblas_int info = 0;
+  do {
+    if (hipMalloc(&dA, sizeof(double) * A.size()) != hipSuccess || hipMalloc(&dD, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&dE, sizeof(double) * n) != hipSuccess || hipMalloc(&dinfo, sizeof(rocblas_int)) != hipSuccess) {
+      rc = fail(ACHR_E_NOMEM, "device allocation for the %d x %d eigenproblem", n, n);
+      break;
+    }
+    if (hipMemcpy(dA, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = fail(ACHR_E_DEVICE, "upload S^T S");
+      break;
+    }
+    rocblas_status s = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, dA, n, dD, dE, dinfo);
+    if (s != rocblas_status_success) {
+      rc = fail(ACHR_E_SOLVER, "rocsolver_dsyevd status %d", static_cast<int>(s));
+      break;
+    }
+    if (hipMemcpy(&info, dinfo, sizeof info, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ev.data(), dD, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(A.data(), dA, sizeof(double) * A.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(ACHR_E_DEVICE, "download eigenpairs");
+      break;
+    }
+    if (info != 0) {
+      rc = fail(ACHR_E_SOLVER, "rocsolver_dsyevd info %d", static_cast<int>(info));
+      break;
+    }
+  } while (false);
+  (void)hipFree(dA);
+  (void)hipFree(dD);
+  (void)hipFree(dE);
+  (void)hipFree(dinfo);
+  rocblas_destroy_handle(h);
+  if (rc != ACHR_OK) return rc;
+  // Eigenvalues ascend; the largest singular value of S^T S is max |lambda|.
+  double smax = 0.0;
+  for (double v : ev) smax = std::max(smax, std::fabs(v));
+  const double tol = static_cast<double>(n) * smax * 2.2204e-16;  // ACHRCuda.cu:41, :474
+  std::vector<int> keep;
+  for (int i = 0; i < n; ++i)
+    if (std::fabs(ev[i]) < tol) keep.push_back(i);
+  // Order the basis like the reference (descending singular value): the
+  // largest of the small eigenvalues first.
+  std::sort(keep.begin(), keep.end(), [&](int a, int b) { return std::fabs(ev[a]) > std::fabs(ev[b]); });
+  const int d = static_cast<int>(keep.size());
+  M->N.assign(static_cast<size_t>(n) * d, 0.0);
+  for (int j = 0; j < d; ++j)
+    std::copy(A.begin() + static_cast<size_t>(keep[j]) * n, A.begin() + static_cast<size_t>(keep[j] + 1) * n,
+              M->N.begin() + static_cast<size_t>(j) * n);
+  M->d = d;
+  if (null_dim) *null_dim = d;
+  return ACHR_OK;
+}
+
+// ---------------------------------------------------------------- sampler
+
+static achr::Schedule pick_schedule(int n, int lanes_req) {
+  achr::Schedule s{};
+  static const int kTpc[] = {16, 32, 64, 128, 256, 512, 1024};
+  if (lanes_req > 0) {
+    s.tpc = lanes_req;
+    s.e = achr::elems_for(n, lanes_req);
+  } else {
+    s.tpc = 1024;
+    s.e = achr::elems_for(n, 1024);
+    for (int t : kTpc) {
+      if ((n + t - 1) / t <= 4) {
+        s.tpc = t;
+        s.e = achr::elems_for(n, t);
+        break;
+      }
+    }
+  }
+  s.cpb = s.tpc <= 64 ? 256 / s.tpc : 1;
+  return s;
+}
+
+void achr_sampler_destroy(achr_sampler* S) {
+  if (!S) return;
+  for (hipEvent_t e : S->ev_pool) (void)hipEventDestroy(e);
+  if (S->ev_begin) (void)hipEventDestroy(S->ev_begin);
+  if (S->ev_end) (void)hipEventDestroy(S->ev_end);
+  delete S;
+}
+
+int achr_sampler_create(const achr_model* M, const double* warmup, int W, const achr_options* opts,
+                        achr_sampler** out) {
+  if (!M || !warmup || !out) return fail(ACHR_E_ARG, "NULL argument");
+  *out = nullptr;
+  if (W <= 0) return fail(ACHR_E_ARG, "no warmup points");
+  if (M->d < 0) return fail(ACHR_E_STATE, "model has no null space: call achr_model_compute_nullspace or set one");
+  achr_options o;
+  achr_options_default(&o);
+  if (opts) o = *opts;
+  if (o.check_every <= 0 || o.max_misses < 0) return fail(ACHR_E_ARG, "bad check_every / max_misses");
+  if (o.lanes != 0 && !achr::schedule_supported(o.lanes, achr::elems_for(M->n, o.lanes)))
+    return fail(ACHR_E_ARG, "lanes=%d cannot hold n=%d reactions", o.lanes, M->n);
+  if (o.trace_chains > 0 && o.trace_thin <= 0) return fail(ACHR_E_ARG, "trace_thin must be > 0");
+  auto* S = new (std::nothrow) achr_sampler();
+  if (!S) return fail(ACHR_E_NOMEM, "sampler allocation");
+  S->n = M->n;
+  S->m = M->m;
+  S->W = W;
+  S->d = M->d;
+  S->opt = o;
+  S->sched = pick_schedule(M->n, o.lanes);
+  // Groups smaller than a wavefront address the warmup set with 32-bit byte
+  // offsets from its base.
+  if (S->sched.tpc < 64 && 8.0 * M->n * W >= 2147483648.0) {
+    if (o.lanes != 0) {
+      delete S;
+      return fail(ACHR_E_ARG, "lanes=%d needs n*W*8 < 2 GiB", o.lanes);
+    }
+    S->sched = pick_schedule(M->n, 64);
+  }
+  if (S->sched.e < 0) {
+    delete S;
+    return fail(ACHR_E_ARG, "n=%d exceeds the largest register schedule (1024 x 16)", M->n);
+  }
+  const double nd = static_cast<double>(M->n) * static_cast<double>(M->d);
+  S->mode = o.reproject_mode != 0 ? o.reproject_mode : (nd <= 65536.0 ? 1 : 2);
+  // Warmup centre on the host: (1/W) * sum_k w_k, k ascending (createCenterPt).
+  const int n = M->n;
+  S->c0.assign(n, 0.0);
+  const double alpha = 1.0 / static_cast<double>(W);
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int k = 0; k < W; ++k) s += warmup[static_cast<size_t>(k) * n + i];
+    S->c0[i] = alpha * s;
+  }
+  std::vector<double> Nrm(static_cast<size_t>(n) * M->d);
+  for (int j = 0; j < M->d; ++j)
+    for (int i = 0; i < n; ++i) Nrm[static_cast<size_t>(i) * M->d + j] = M->N[static_cast<size_t>(j) * n + i];
+  hipStream_t s0 = nullptr;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = S->warm.upload(warmup, static_cast<size_t>(n) * W, s0);
+  if (e == hipSuccess) e = S->lb.upload(M->lb.data(), n, s0);
+  if (e == hipSuccess) e = S->ub.upload(M->ub.data(), n, s0);
+  if (e == hipSuccess) e = S->c0d.upload(S->c0.data(), n, s0);
+  if (e == hipSuccess) e = S->rowptr.upload(M->rowptr.data(), M->rowptr.size(), s0);
+  if (e == hipSuccess) e = S->colidx.upload(M->colidx.data(), M->colidx.size(), s0);
+  if (e == hipSuccess) e = S->vals.upload(M->vals.data(), M->vals.size(), s0);
+  if (e == hipSuccess) e = S->Ncm.upload(M->N.data(), M->N.size(), s0);
+  if (e == hipSuccess) e = S->Nrm.upload(Nrm.data(), Nrm.size(), s0);
+  if (e == hipSuccess) e = hipEventCreate(&S->ev_begin);
+  if (e == hipSuccess) e = hipEventCreate(&S->ev_end);
+  if (e != hipSuccess) {
+    achr_sampler_destroy(S);
+    return fail(ACHR_E_DEVICE, "sampler upload: %s", hipGetErrorString(e));
+  }
+  *out = S;
+  return ACHR_OK;
+}
+
+int achr_read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_out) {
+  if (!path || !warmup || !W_out) return fail(ACHR_E_ARG, "NULL argument");
+  *warmup = nullptr;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(ACHR_E_IO, "cannot open warmup CSV %s", path);
+  std::vector<std::vector<double>> rows;
+  std::string line;
+  std::vector<char> buf(1 << 20);
+  int W = -1;
+  int rc = ACHR_OK;
+  while (true) {
+    line.clear();
+    bool got = false;
+    while (std::fgets(buf.data(), static_cast<int>(buf.size()), f)) {
+      got = true;
+      line += buf.data();
+      if (!line.empty() && line.back() == '\n') break;
+    }
+    if (!got) break;
+    while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
+    if (line.empty()) continue;
+    std::vector<double> vals;
+    const char* p = line.c_str();
+    while (true) {
+      char* end = nullptr;
+      const double v = std::strtod(p, &end);
+      if (end == p) { rc = fail(ACHR_E_FORMAT, "%s: line %zu: bad number", path, rows.size() + 1); break; }
+      vals.push_back(v);
+      while (*end == ' ' || *end == '\t') ++end;
+      if (*end == ',') { p = end + 1; continue; }
+      if (*end == '\0') break;
+      rc = fail(ACHR_E_FORMAT, "%s: line %zu: unexpected character", path, rows.size() + 1);
+      break;
+    }
+    if (rc != ACHR_OK) break;
+    if (W < 0) W = static_cast<int>(vals.size());
+    if (static_cast<int>(vals.size()) != W) {
+      rc = fail(ACHR_E_FORMAT, "%s: line %zu has %zu values, expected %d", path, rows.size() + 1, vals.size(), W);
+      break;
+    }
+    rows.push_back(std::move(vals));
+  }
+  std::fclose(f);
+  if (rc != ACHR_OK) return rc;
+  const int n = static_cast<int>(rows.size());
+  if (n == 0 || W <= 0) return fail(ACHR_E_FORMAT, "%s: empty warmup file", path);
+  if (n_expected > 0 && n != n_expected)
+    return fail(ACHR_E_FORMAT, "%s: %d lines but the model has %d reactions", path, n, n_expected);
+  auto* w = static_cast<double*>(std::malloc(sizeof(double) * static_cast<size_t>(n) * W));
+  if (!w) return fail(ACHR_E_NOMEM, "warmup allocation");
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < W; ++k) w[static_cast<size_t>(k) * n + i] = rows[i][k];
+  *warmup = w;
+  *W_out = W;
+  return ACHR_OK;
+}
+
+int achr_sampler_create_csv(const achr_model* M, const char* path, const achr_options* opts,
+                            achr_sampler** out) {
+  if (!M) return fail(ACHR_E_ARG, "model is NULL");
+  double* w = nullptr;
+  int W = 0;
+  int rc = achr_read_warmup_csv(path, M->n, &w, &W);
+  if (rc != ACHR_OK) return rc;
+  rc = achr_sampler_create(M, w, W, opts, out);
+  std::free(w);
+  return rc;
+}
+
+int achr_sampler_set_stream(achr_sampler* S, void* stream) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  S->stream = static_cast<hipStream_t>(stream);
+  return ACHR_OK;
+}
+
+int achr_sampler_center(const achr_sampler* S, double* c0) {
+  if (!S || !c0) return fail(ACHR_E_ARG, "NULL argument");
+  std::copy(S->c0.begin(), S->c0.end(), c0);
+  return ACHR_OK;
+}
+
+int achr_sampler_schedule(const achr_sampler* S, int* lanes, int* mode) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  if (lanes) *lanes = S->sched.tpc;
+  if (mode) *mode = S->mode;
+  return ACHR_OK;
+}
+
+static int event_at(achr_sampler* S, size_t i, hipEvent_t* ev) {
+  while (S->ev_pool.size() <= i) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    S->ev_pool.push_back(e);
+  }
+  *ev = S->ev_pool[i];
+  return ACHR_OK;
+}
+
+int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  if (P <= 0 || Ssteps < 0 || chain0 < 0) return fail(ACHR_E_ARG, "bad P=%d S=%d chain0=%d", P, Ssteps, chain0);
+  const int n = S->n;
+  const size_t Pn = static_cast<size_t>(P) * n;
+  if (S->sched.tpc < 64) {
+    const double lim = 2147483648.0;
+    const double trace_rows = static_cast<double>(std::min(S->opt.trace_chains, P)) *
+                              (S->opt.trace_thin > 0 ? Ssteps / S->opt.trace_thin : 0);
+    if (8.0 * P * n >= lim || 8.0 * P * std::max(S->d, 1) >= lim || 8.0 * trace_rows * n >= lim)
+      return fail(ACHR_E_ARG, "P=%d too large for a %d-lane schedule (32-bit row offsets)", P, S->sched.tpc);
+  }
+  const int tr_chains = std::min(S->opt.trace_chains, P);
+  const int tr_len = S->opt.trace_thin > 0 ? Ssteps / S->opt.trace_thin : 0;
+  HIP_TRY(S->X.ensure(Pn));
+  HIP_TRY(S->C.ensure(Pn));
+  HIP_TRY(S->T.ensure(static_cast<size_t>(P) * std::max(S->d, 1)));
+  HIP_TRY(S->out.ensure(Pn));
+  HIP_TRY(S->st.ensure(P));
+  HIP_TRY(S->stats.ensure(P));
+  HIP_TRY(S->flag.ensure(P));
+  HIP_TRY(S->idx.ensure(P));
+  HIP_TRY(S->count.ensure(1));
+  if (tr_chains > 0 && tr_len > 0) HIP_TRY(S->trace.ensure(static_cast<size_t>(tr_chains) * tr_len * n));
+  S->P = P;
+  S->S = Ssteps;
+  S->trace_len = (tr_chains > 0) ? tr_len : 0;
+  hipStream_t st = S->stream;
+  HIP_TRY(hipMemsetAsync(S->st.p, 0, sizeof(int4) * P, st));
+  HIP_TRY(hipMemsetAsync(S->stats.p, 0, sizeof(int4) * P, st));
+  HIP_TRY(hipMemsetAsync(S->flag.p, 0, sizeof(int) * P, st));
+
+  achr::StepArgs a{};
+  a.n = n;
+  a.m = S->m;
+  a.W = S->W;
+  a.d = S->d;
+  a.warm = S->warm.p;
+  a.lb = S->lb.p;
+  a.ub = S->ub.p;
+  a.c0 = S->c0d.p;
+  a.rowptr = S->rowptr.p;
+  a.colidx = S->colidx.p;
+  a.vals = S->vals.p;
+  a.Ncm = S->Ncm.p;
+  a.Nrm = S->Nrm.p;
+  a.P = P;
+  a.S = Ssteps;
+  a.seed = seed;
+  a.file = file;
+  a.model = 0;
+  a.chain0 = chain0;
+  a.dTol = S->opt.dTol;
+  a.uTol = S->opt.uTol;
+  a.maxMinTol = S->opt.maxMinTol;
+  a.reprojTol = S->opt.reprojTol;
+  a.check_every = S->opt.check_every;
+  a.max_misses = S->opt.max_misses;
+  a.mode = S->mode;
+  a.X = S->X.p;
+  a.C = S->C.p;
+  a.T = S->T.p;
+  a.st = S->st.p;
+  a.stats = S->stats.p;
+  a.flag = S->flag.p;
+  a.out = S->out.p;
+  a.trace = S->trace_len > 0 ? S->trace.p : nullptr;
+  a.trace_chains = tr_chains;
+  a.trace_thin = S->opt.trace_thin;
+  a.trace_len = S->trace_len;
+
+  S->ev_step.clear();
+  S->ev_proj.clear();
+  size_t evn = 0;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventRecord(S->ev_begin, st));
+  const int checks = (Ssteps + S->opt.check_every - 1) / S->opt.check_every;
+  const int segments = (S->mode == 2) ? checks + 1 : 1;
+  for (int seg = 0; seg < segments; ++seg) {
+    int rc = event_at(S, evn, &e0);
+    if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
+    if (rc != ACHR_OK) return rc;
+    HIP_TRY(hipEventRecord(e0, st));
+    hipError_t le = achr::launch_step(a, S->sched, st);
+    if (le != hipSuccess) return fail(ACHR_E_DEVICE, "step kernel launch: %s", hipGetErrorString(le));
+    HIP_TRY(hipEventRecord(e1, st));
+    S->ev_step.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
+    evn += 2;
+    if (S->mode == 2 && seg + 1 < segments && S->d > 0) {
+      rc = event_at(S, evn, &e0);
+      if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
+      if (rc != ACHR_OK) return rc;
+      HIP_TRY(hipEventRecord(e0, st));
+      le = achr::launch_compact(S->flag.p, P, S->idx.p, S->count.p, st);
+      if (le == hipSuccess) le = achr::launch_project(S->Ncm.p, n, S->d, S->X.p, S->T.p, S->idx.p, S->count.p, P, st);
+      if (le != hipSuccess) return fail(ACHR_E_DEVICE, "projection launch: %s", hipGetErrorString(le));
+      HIP_TRY(hipEventRecord(e1, st));
+      S->ev_proj.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
+      evn += 2;
+    }
+  }
+  HIP_TRY(hipEventRecord(S->ev_end, st));
+  S->have_run = true;
+  return ACHR_OK;
+}
+
+int achr_sample_points(achr_sampler* S, double* out) {
+  if (!S || !out) return fail(ACHR_E_ARG, "NULL argument");
+  if (!S->have_run) return fail(ACHR_E_STATE, "no run");
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  HIP_TRY(hipMemcpy(out, S->out.p, sizeof(double) * static_cast<size_t>(S->P) * S->n, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
+int achr_sample_points_device(achr_sampler* S, void** dev_ptr) {
+  if (!S || !dev_ptr) return fail(ACHR_E_ARG, "NULL argument");
+  if (!S->have_run) return fail(ACHR_E_STATE, "no run");
+  *dev_ptr = S->out.p;
+  return ACHR_OK;
+}
+
+int achr_sample_stats(achr_sampler* S, int32_t* stats) {
+  if (!S || !stats) return fail(ACHR_E_ARG, "NULL argument");
+  if (!S->have_run) return fail(ACHR_E_STATE, "no run");
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  HIP_TRY(hipMemcpy(stats, S->stats.p, sizeof(int4) * S->P, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
+int achr_sample_trace(achr_sampler* S, double* trace) {
+  if (!S || !trace) return fail(ACHR_E_ARG, "NULL argument");
+  if (!S->have_run) return fail(ACHR_E_STATE, "no run");
+  if (S->trace_len == 0) return fail(ACHR_E_STATE, "no trace recorded (trace_chains / trace_thin)");
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  const size_t cnt = static_cast<size_t>(std::min(S->opt.trace_chains, S->P)) * S->trace_len * S->n;
+  HIP_TRY(hipMemcpy(trace, S->trace.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
+int achr_sample_timing(achr_sampler* S, double* ms_total, double* ms_step, double* ms_proj, int* n_step,
+                       int* n_proj) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  if (!S->have_run) return fail(ACHR_E_STATE, "no run");
+  HIP_TRY(hipEventSynchronize(S->ev_end));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, S->ev_begin, S->ev_end));
+  double tstep = 0.0, tproj = 0.0;
+  for (auto& pr : S->ev_step) {
+    HIP_TRY(hipEventElapsedTime(&ms, S->ev_pool[pr.first], S->ev_pool[pr.second]));
+    tstep += ms;
+  }
+  for (auto& pr : S->ev_proj) {
+    HIP_TRY(hipEventElapsedTime(&ms, S->ev_pool[pr.first], S->ev_pool[pr.second]));
+    tproj += ms;
+  }
+  HIP_TRY(hipEventElapsedTime(&ms, S->ev_begin, S->ev_end));
+  if (ms_total) *ms_total = ms;
+  if (ms_step) *ms_step = tstep;
+  if (ms_proj) *ms_proj = tproj;
+  if (n_step) *n_step = static_cast<int>(S->ev_step.size());
+  if (n_proj) *n_proj = static_cast<int>(S->ev_proj.size());
+  return ACHR_OK;
+}
+
+int achr_write_points_csv(const char* path, const double* points, int n, int P) {
+  if (!path || !points || n <= 0 || P <= 0) return fail(ACHR_E_ARG, "bad arguments");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(ACHR_E_IO, "cannot open %s for writing", path);
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < P - 1; ++j) std::fprintf(f, "%f,", points[static_cast<size_t>(i) * P + j]);
+    std::fprintf(f, "%f\n", points[static_cast<size_t>(i) * P + (P - 1)]);
+  }
+  if (std::fclose(f) != 0) return fail(ACHR_E_IO, "write error on %s", path);
+  return ACHR_OK;
+}
+
+}  // extern "C"

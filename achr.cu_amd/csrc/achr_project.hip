@@ -1,0 +1,161 @@
+// Batched FP64 null-space reprojection X <- N (N^T X) over the chains that
+// asked for it in the current round.
+//
+// The reference reprojects one chain at a time with two GEMVs
+// (reprojectPoint / reprojectPoint2, ACHRcu/ACHRCuda.cu:208-230), which at
+// genome scale reads the whole of N (n x d doubles) per chain per check.
+// Here the flagged chains of a round are gathered and both products run as
+// register-tiled FP64 GEMMs, so N is read once per 64 chains. Each output is
+// accumulated with fused multiply-adds in increasing k, starting from +0.0 —
+// the loop order of the reference's two kernels — so the result is bitwise
+// the sequential dot product the host oracle computes. (On gfx950 FP64 vector
+// FMA has the same peak as FP64 MFMA, so nothing is lost by staying on the
+// VALU with a fixed accumulation order.)
+#include "achr_internal.h"
+
+namespace achr {
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16, TM = 4, TN = 4;
+constexpr int kThreads = (BM / TM) * (BN / TN);  // 256
+
+// C(r, q) = sum_k A(r, k) * B(k, q) for r < M, k < K, q < *count.
+// A(r, k) = A[r*sar + k*sak]; column q of B starts at B + colB(q)*ldb and is
+// contiguous in k; column q of C starts at Cm + colC(q)*ldc, contiguous in r.
+// colB(q) = idxB ? idxB[q] : q, likewise colC.
+__global__ __launch_bounds__(kThreads) void dgemm_seq_kernel(int M, int K, const double* __restrict__ A,
+                                                              long sar, long sak, const double* __restrict__ B,
+                                                              long ldb, const int* __restrict__ idxB,
+                                                              double* __restrict__ Cm, long ldc,
+                                                              const int* __restrict__ idxC,
+                                                              const int* __restrict__ count) {
+  const int ncols = *count;
+  const int q0 = blockIdx.y * BN;
+  if (q0 >= ncols) return;
+  const int r0 = blockIdx.x * BM;
+  __shared__ double As[BK][BM + 2];
+  __shared__ double Bs[BK][BN + 2];
+  const int t = threadIdx.x;
+  const int ty = t / (BN / TN), tx = t % (BN / TN);
+  double acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = 0.0;
+
+  // Column base pointers of the B columns this thread loads.
+  const double* bcol[(BK * BN) / kThreads];
+  int bk[(BK * BN) / kThreads], bq[(BK * BN) / kThreads];
+#pragma unroll
+  for (int e = 0; e < (BK * BN) / kThreads; ++e) {
+    const int lin = t + e * kThreads;
+    const int q = lin / BK, k = lin % BK;
+    bq[e] = q;
+    bk[e] = k;
+    const int gq = q0 + q;
+    bcol[e] = gq < ncols ? B + (idxB ? static_cast<long>(idxB[gq]) : static_cast<long>(gq)) * ldb : nullptr;
+  }
+  const bool a_kcontig = (sak == 1);
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int e = 0; e < (BK * BM) / kThreads; ++e) {
+      const int lin = t + e * kThreads;
+      int r, k;
+      if (a_kcontig) {
+        r = lin / BK;
+        k = lin % BK;
+      } else {
+        r = lin % BM;
+        k = lin / BM;
+      }
+      const int gr = r0 + r, gk = k0 + k;
+      As[k][r] = (gr < M && gk < K) ? A[static_cast<long>(gr) * sar + static_cast<long>(gk) * sak] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < (BK * BN) / kThreads; ++e) {
+      const int gk = k0 + bk[e];
+      Bs[bk[e]][bq[e]] = (bcol[e] != nullptr && gk < K) ? bcol[e][gk] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BK; ++k) {
+      double av[TM], bv[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) av[i] = As[k][ty * TM + i];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bv[j] = Bs[k][tx * TN + j];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int gq = q0 + tx * TN + j;
+    if (gq >= ncols) continue;
+    double* ccol = Cm + (idxC ? static_cast<long>(idxC[gq]) : static_cast<long>(gq)) * ldc;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int gr = r0 + ty * TM + i;
+      if (gr < M) ccol[gr] = acc[i][j];
+    }
+  }
+}
+
+// Stable compaction of flag[0..P) into idx (increasing chain order); one block.
+__global__ __launch_bounds__(1024) void compact_kernel(const int* __restrict__ flag, int P, int* __restrict__ idx,
+                                                        int* __restrict__ count) {
+  __shared__ int wsum[16];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < P; c0 += 1024) {
+    const int i = c0 + threadIdx.x;
+    const int f = (i < P && flag[i]) ? 1 : 0;
+    // inclusive scan within the wavefront
+    int v = f;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(v, o);
+      if (lane >= o) v += y;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    int off = base;
+    for (int j = 0; j < w; ++j) off += wsum[j];
+    if (f) idx[off + v - 1] = i;
+    __syncthreads();
+    if (threadIdx.x == 1023) base = off + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = base;
+}
+
+}  // namespace
+
+hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStream_t stream) {
+  hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, stream, flag, P, idx, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_project(const double* Ncm, int n, int d, double* X, double* T, const int* idx,
+                          const int* count, int P, hipStream_t stream) {
+  const unsigned gq = static_cast<unsigned>((P + BN - 1) / BN);
+  // T(:, q) = N^T X(:, idx[q]):   A(j, i) = N[i + j*n]  (sar = n, sak = 1)
+  hipLaunchKernelGGL(dgemm_seq_kernel, dim3((d + BM - 1) / BM, gq), dim3(kThreads), 0, stream, d, n, Ncm,
+                     static_cast<long>(n), 1L, X, static_cast<long>(n), idx, T, static_cast<long>(d),
+                     static_cast<const int*>(nullptr), count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // X(:, idx[q]) = N T(:, q):     A(i, j) = N[i + j*n]  (sar = 1, sak = n)
+  hipLaunchKernelGGL(dgemm_seq_kernel, dim3((n + BM - 1) / BM, gq), dim3(kThreads), 0, stream, n, d, Ncm, 1L,
+                     static_cast<long>(n), T, static_cast<long>(d), static_cast<const int*>(nullptr), X,
+                     static_cast<long>(n), idx, count);
+  return hipGetLastError();
+}
+
+}  // namespace achr

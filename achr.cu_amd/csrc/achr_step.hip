@@ -1,0 +1,444 @@
+// FP64 ACHR chain-step kernel for gfx950.
+//
+// One chain is owned by a group of TPC threads (TPC = 16..64: a slice of one
+// wavefront, several chains per workgroup; TPC = 128..1024: one workgroup per
+// chain). Element i of the n-vector belongs to thread i % TPC, slot i / TPC,
+// and the chain's point x and running centre c stay in registers for the
+// whole launch; per accepted step the only HBM traffic is the gathered warmup
+// column (n doubles). The loop body restates createPoint/fillrandPoint/
+// advNextStep/findMaxAbs/reprojectPoint(2)/correctBounds of the reference
+// (ACHRcu/ACHRCuda.cu:193-359) with the reference's tolerances and order:
+//   pick w -> u = (w - c)/|w - c| -> chord [minStep, maxStep] from the bound
+//   distances of coordinates farther than dTol from both bounds and with
+//   |u_i| > uTol -> miss test -> x += t u -> every check_every accepted steps
+//   max|S x| > reprojTol => x = N (N^T x) -> clip -> centre update.
+// Reductions: the squared norm uses the canonical lane tree (shuffle xor
+// within 64 lanes, then a pairwise tree over the wavefront partials); minima
+// and maxima are order independent.
+//
+// Two ways to run the reprojection:
+//   mode 1: inside this kernel, per chain (N read through L2; for small n*d);
+//   mode 2: the kernel stops after the advance of every check step, flags the
+//           chains whose residual exceeds the tolerance, and the host enqueues
+//           a batched projection GEMM over the flagged chains before the next
+//           segment resumes them (chains stay aligned on accepted steps, so a
+//           round of check_every steps is one segment for every chain).
+#include <math.h>
+
+#include "achr_internal.h"
+#include "achr_philox.h"
+
+namespace achr {
+namespace {
+
+constexpr int kRedSlots = 16;  // wavefront partials per reduction (<= 1024 threads)
+
+// Buffer-resource loads: 32-bit byte offsets (one VGPR per slot, shared by
+// every per-reaction array) and hardware range checking — an offset at or
+// beyond num_records reads 0.0 without touching memory. Zero is inert in
+// every formula of the step (u = 0 adds nothing to the norm and is below
+// uTol; a zero x with zero bounds is not farther than dTol from them, stays
+// unclipped, and keeps a zero centre), so slots past n need no masks.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, gfx9-family data format
+constexpr int kOob = 0x7FFFFFF8;        // an offset every resource here rejects
+
+__device__ __forceinline__ rsrc_t make_rsrc(const double* p, long long bytes) {
+  const int nrec = bytes > 0x7FFFFFF0LL ? 0x7FFFFFF0 : static_cast<int>(bytes);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), static_cast<short>(0), nrec, kRsrcWord3);
+}
+
+__device__ __forceinline__ double bload(rsrc_t r, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+template <int TPC>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (TPC > 64) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+template <int TPC>
+__device__ __forceinline__ double group_sum(double v, double* red) {
+  constexpr int G = TPC < 64 ? TPC : 64;
+#pragma unroll
+  for (int o = G / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o);
+  if constexpr (TPC > 64) {
+    constexpr int NW = TPC / 64;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double p[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) p[i] = red[i];
+#pragma unroll
+    for (int o = NW / 2; o >= 1; o >>= 1) {
+#pragma unroll
+      for (int i = 0; i < o; ++i) p[i] = p[i] + p[i + o];
+    }
+    v = p[0];
+  }
+  return v;
+}
+
+template <int TPC>
+__device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red_mn, double* red_mx) {
+  constexpr int G = TPC < 64 ? TPC : 64;
+#pragma unroll
+  for (int o = G / 2; o >= 1; o >>= 1) {
+    const double a = __shfl_xor(mn, o);
+    const double b = __shfl_xor(mx, o);
+    if (a > mn) mn = a;
+    if (b < mx) mx = b;
+  }
+  if constexpr (TPC > 64) {
+    constexpr int NW = TPC / 64;
+    if ((threadIdx.x & 63) == 0) {
+      red_mn[threadIdx.x >> 6] = mn;
+      red_mx[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      const double a = red_mn[i], b = red_mx[i];
+      if (a > mn) mn = a;
+      if (b < mx) mx = b;
+    }
+  }
+}
+
+template <int TPC>
+__device__ __forceinline__ double group_max(double v, double* red) {
+  constexpr int G = TPC < 64 ? TPC : 64;
+#pragma unroll
+  for (int o = G / 2; o >= 1; o >>= 1) {
+    const double a = __shfl_xor(v, o);
+    if (a > v) v = a;
+  }
+  if constexpr (TPC > 64) {
+    constexpr int NW = TPC / 64;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      const double a = red[i];
+      if (a > v) v = a;
+    }
+  }
+  return v;
+}
+
+__device__ __forceinline__ void bstore(rsrc_t r, int off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned int, v), r,
+                                        off, 0, 0);
+}
+
+// Addressing of one n-vector per chain (the chain's row of X, T, a trace
+// row): with TPC >= 64 the chain index is wave-uniform and the row gets its
+// own resource; smaller groups share a wavefront with other chains, so they
+// address from the array base with a per-lane offset (the host keeps such
+// arrays under 2 GiB) and send slots past the row length out of range.
+template <int TPC>
+struct Row {
+  rsrc_t r;
+  int base;  // element offset of the row from the resource base
+  int len;
+  __device__ __forceinline__ Row(double* arr, long long row, int ld, int len_) : len(len_) {
+    if constexpr (TPC >= 64) {
+      const long long ru = __builtin_amdgcn_readfirstlane(static_cast<int>(row));
+      r = make_rsrc(arr + ru * ld, 8LL * len_);
+      base = 0;
+    } else {
+      r = make_rsrc(arr, 0x7FFFFFF0LL);
+      base = static_cast<int>(row) * ld;
+    }
+  }
+  __device__ __forceinline__ int off(int i) const {
+    if constexpr (TPC >= 64) {
+      return i * 8;
+    } else {
+      return i < len ? (base + i) * 8 : kOob;
+    }
+  }
+  __device__ __forceinline__ double ld(int i) const { return bload(r, off(i)); }
+  __device__ __forceinline__ void st(int i, double v) const { bstore(r, off(i), v); }
+};
+
+template <int TPC, int E, int CPB>
+__global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
+  extern __shared__ double lds[];
+  double* red = lds;  // 4 reduction slot arrays; only block-per-chain schedules use them
+  const int g = threadIdx.x / TPC;
+  const int lane = threadIdx.x % TPC;
+  const int b = blockIdx.x * CPB + g;
+  if (b >= a.P) return;  // group-uniform; with TPC > 64 the block is one chain
+  if (a.mode == 2 && lane == 0) a.flag[b] = 0;
+  int4 st = a.st[b];
+  if (st.z == kDone) return;
+  int4 sts = a.stats[b];
+  const int n = a.n;
+  const int d = a.d;
+
+  const rsrc_t r_lb = make_rsrc(a.lb, 8LL * n);
+  const rsrc_t r_ub = make_rsrc(a.ub, 8LL * n);
+  const Row<TPC> xrow(a.X, b, n, n);   // spill of x for the residual / projection
+  const Row<TPC> crow(a.C, b, n, n);
+  const Row<TPC> trow(a.T, b, d, d);   // N^T x of the in-kernel projection
+
+  // Large slot counts keep the running centre in LDS (its own slots only, so
+  // no barrier is needed); x and the direction stay in registers.
+  constexpr bool kCL = E > 8;
+  double x[E];
+  double c[kCL ? 1 : E];
+  double* cl = lds + 4 * kRedSlots;
+  if (st.z == kFresh) {
+    const rsrc_t r_c0 = make_rsrc(a.c0, 8LL * n);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const double v = bload(r_c0, (lane + k * TPC) * 8);
+      x[k] = v;
+      if constexpr (kCL) cl[lane + k * TPC] = v; else c[k] = v;
+    }
+    st = make_int4(0, 0, kRunning, 0);
+  } else {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      x[k] = xrow.ld(lane + k * TPC);
+      const double cv = crow.ld(lane + k * TPC);
+      if constexpr (kCL) cl[lane + k * TPC] = cv; else c[k] = cv;
+    }
+  }
+
+  // Small models address the warmup set from its base (the host only picks
+  // such schedules when W*n*8 < 2^31).
+  const rsrc_t r_wall = make_rsrc(a.warm, 8LL * n * a.W);
+
+  uint32_t attempt = static_cast<uint32_t>(st.x);
+  int s = st.y;
+  int consec = st.w;
+  bool pending = st.z == kPendingCheck;
+  bool stopped_at_check = false;
+  const uint32_t chain = static_cast<uint32_t>(a.chain0 + b);
+
+  for (;;) {
+    if (!pending) {
+      if (s >= a.S) break;
+      const double up = stream_u01(a.seed, attempt, chain, a.file, a.model, kTagPick);
+      ++attempt;
+      const int id = pick_index(up, a.W);
+      double u[E];
+      double ss = 0.0;
+      if constexpr (TPC >= 64) {
+        // id is uniform over the wavefront: a per-column resource in SGPRs.
+        const int idu = __builtin_amdgcn_readfirstlane(id);
+        const rsrc_t r_w = make_rsrc(a.warm + static_cast<size_t>(idu) * n, 8LL * n);
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          u[k] = bload(r_w, (lane + k * TPC) * 8) - (kCL ? cl[lane + k * TPC] : c[k]);
+          ss = fma(u[k], u[k], ss);
+        }
+      } else {
+        const int wbase = id * n;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const int i = lane + k * TPC;
+          u[k] = bload(r_wall, i < n ? (wbase + i) * 8 : kOob) - (kCL ? cl[i] : c[k]);
+          ss = fma(u[k], u[k], ss);
+        }
+      }
+      ss = group_sum<TPC>(ss, red);
+      const double nrm = sqrt(ss);
+      double mn = -INFINITY, mx = INFINITY;
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int off = (lane + k * TPC) * 8;
+        const double ui = u[k] / nrm;
+        u[k] = ui;
+        const double du = bload(r_ub, off) - x[k];
+        const double dl = x[k] - bload(r_lb, off);
+        if (du > a.dTol && dl > a.dTol) {
+          if (ui > a.uTol) {
+            double v = -dl / ui;
+            if (v > mn) mn = v;
+            v = du / ui;
+            if (v < mx) mx = v;
+          } else if (ui < -a.uTol) {
+            double v = du / ui;
+            if (v > mn) mn = v;
+            v = -dl / ui;
+            if (v < mx) mx = v;
+          }
+        }
+        // Keep the schedule slot by slot: hoisting every slot's bound loads
+        // and FP64 divisions ahead multiplies the live registers by E.
+        if constexpr (E > 4) __builtin_amdgcn_sched_barrier(0);
+      }
+      group_minmax<TPC>(mn, mx, red + kRedSlots, red + 2 * kRedSlots);
+      const bool miss = !(nrm > 0.0) || !(mn > -INFINITY) || !(mx < INFINITY) ||
+                        (fabs(mn) < a.maxMinTol && fabs(mx) < a.maxMinTol) || (mn > mx);
+      if (miss) {
+        ++sts.x;
+        if (++consec > a.max_misses) {
+          sts.w |= 1;
+          break;
+        }
+        continue;
+      }
+      consec = 0;
+      const double rs = stream_u01(a.seed, static_cast<uint32_t>(s), chain, a.file, a.model, kTagStep);
+      const double t = fma(rs, mx - mn, mn);
+#pragma unroll
+      for (int k = 0; k < E; ++k) x[k] = fma(t, u[k], x[k]);
+
+      if (s % a.check_every == 0) {
+        ++sts.z;
+#pragma unroll
+        for (int k = 0; k < E; ++k) xrow.st(lane + k * TPC, x[k]);
+        group_sync<TPC>();
+        double rho = 0.0;
+        for (int j = lane; j < a.m; j += TPC) {
+          double acc = 0.0;
+          const int p1 = a.rowptr[j + 1];
+          for (int p = a.rowptr[j]; p < p1; ++p) acc = fma(a.vals[p], xrow.ld(a.colidx[p]), acc);
+          const double av = fabs(acc);
+          if (av > rho) rho = av;
+        }
+        rho = group_max<TPC>(rho, red + 3 * kRedSlots);
+        const bool reproj = rho > a.reprojTol && d > 0;
+        if (reproj) ++sts.y;
+        if (a.mode == 2) {
+          if (lane == 0) a.flag[b] = reproj ? 1 : 0;
+          stopped_at_check = true;
+          break;
+        }
+        if (reproj) {
+          for (int j = lane; j < d; j += TPC) {
+            double acc = 0.0;
+            const double* Nr = a.Nrm + j;
+            for (int i = 0; i < n; ++i) acc = fma(Nr[static_cast<size_t>(i) * d], xrow.ld(i), acc);
+            trow.st(j, acc);
+          }
+          group_sync<TPC>();
+          // x_i = sum_j N[i, j] tmp_j, j ascending: column j of N is one
+          // resource, its slots are this thread's rows.
+#pragma unroll
+          for (int k = 0; k < E; ++k) x[k] = 0.0;
+          for (int j = 0; j < d; ++j) {
+            const rsrc_t r_nc = make_rsrc(a.Ncm + static_cast<size_t>(j) * n, 8LL * n);
+            const double tj = trow.ld(j);
+#pragma unroll
+            for (int k = 0; k < E; ++k) x[k] = fma(bload(r_nc, (lane + k * TPC) * 8), tj, x[k]);
+          }
+          group_sync<TPC>();
+        }
+      }
+    }
+    pending = false;
+    const double alpha = static_cast<double>(a.W + s) / static_cast<double>(a.W + s + 1);
+    const double beta = 1.0 / static_cast<double>(a.W + s + 1);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int off = (lane + k * TPC) * 8;
+      const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
+      if (x[k] > ubi) x[k] = ubi;
+      else if (x[k] < lbi) x[k] = lbi;
+      if constexpr (kCL) {
+        cl[lane + k * TPC] = fma(alpha, cl[lane + k * TPC], beta * x[k]);
+      } else {
+        c[k] = fma(alpha, c[k], beta * x[k]);
+      }
+      if constexpr (E > 4) __builtin_amdgcn_sched_barrier(0);
+    }
+    ++s;
+    if (a.trace != nullptr && b < a.trace_chains && s % a.trace_thin == 0) {
+      const int kk = s / a.trace_thin - 1;
+      if (kk < a.trace_len) {
+        const Row<TPC> tr(a.trace, static_cast<long long>(b) * a.trace_len + kk, n, n);
+#pragma unroll
+        for (int k = 0; k < E; ++k) tr.st(lane + k * TPC, x[k]);
+      }
+    }
+  }
+
+  if (stopped_at_check) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) crow.st(lane + k * TPC, kCL ? cl[lane + k * TPC] : c[k]);
+    st = make_int4(static_cast<int>(attempt), s, kPendingCheck, consec);
+  } else {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = lane + k * TPC;
+      if (i < n) a.out[static_cast<size_t>(i) * a.P + b] = x[k];
+    }
+    st = make_int4(static_cast<int>(attempt), s, kDone, consec);
+  }
+  if (lane == 0) {
+    a.st[b] = st;
+    a.stats[b] = sts;
+  }
+}
+
+template <int TPC, int E>
+hipError_t launch_tpc_e(const StepArgs& a, hipStream_t stream, size_t lds) {
+  constexpr int CPB = TPC <= 64 ? 256 / TPC : 1;
+  const unsigned grid = static_cast<unsigned>((a.P + CPB - 1) / CPB);
+  hipLaunchKernelGGL((step_kernel<TPC, E, CPB>), dim3(grid), dim3(TPC * CPB), lds, stream, a);
+  return hipGetLastError();
+}
+
+template <int TPC>
+hipError_t launch_tpc(const StepArgs& a, int e, hipStream_t stream, size_t lds) {
+  switch (e) {
+    case 1: return launch_tpc_e<TPC, 1>(a, stream, lds);
+    case 2: return launch_tpc_e<TPC, 2>(a, stream, lds);
+    case 4: return launch_tpc_e<TPC, 4>(a, stream, lds);
+    case 8: return launch_tpc_e<TPC, 8>(a, stream, lds);
+    case 12: return launch_tpc_e<TPC, 12>(a, stream, lds);
+    case 16: return launch_tpc_e<TPC, 16>(a, stream, lds);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+constexpr int kElems[] = {1, 2, 4, 8, 12, 16};
+
+}  // namespace
+
+bool schedule_supported(int tpc, int e) {
+  bool tpc_ok = tpc == 16 || tpc == 32 || tpc == 64 || tpc == 128 || tpc == 256 || tpc == 512 || tpc == 1024;
+  bool e_ok = false;
+  for (int v : kElems) e_ok = e_ok || v == e;
+  return tpc_ok && e_ok;
+}
+
+int elems_for(int n, int tpc) {
+  const int need = (n + tpc - 1) / tpc;
+  for (int v : kElems)
+    if (v >= need) return v;
+  return -1;
+}
+
+size_t step_lds_bytes(const StepArgs&, const Schedule& s) {
+  size_t bytes = 4 * kRedSlots * sizeof(double);
+  if (s.e > 8) bytes += static_cast<size_t>(s.tpc) * s.e * sizeof(double);  // centre in LDS
+  return bytes;
+}
+
+hipError_t launch_step(const StepArgs& a, const Schedule& s, hipStream_t stream) {
+  if (!schedule_supported(s.tpc, s.e)) return hipErrorInvalidValue;
+  const size_t lds = step_lds_bytes(a, s);
+  switch (s.tpc) {
+    case 16: return launch_tpc<16>(a, s.e, stream, lds);
+    case 32: return launch_tpc<32>(a, s.e, stream, lds);
+    case 64: return launch_tpc<64>(a, s.e, stream, lds);
+    case 128: return launch_tpc<128>(a, s.e, stream, lds);
+    case 256: return launch_tpc<256>(a, s.e, stream, lds);
+    case 512: return launch_tpc<512>(a, s.e, stream, lds);
+    case 1024: return launch_tpc<1024>(a, s.e, stream, lds);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace achr

@@ -1,0 +1,189 @@
+/*
+ * achr.h — C ABI of the MI355X ACHR sampler (libachr.so).
+ *
+ * The reference has no library API: its only boundary is the CLI
+ *   ACHRCuda model.mps warmup.csv nFiles pointsPerFile stepsPerPoint
+ * (marouenbg/ACHR.cu ACHRcu/ACHRCuda.cu:552-554) and, inside it, the
+ * device entry stepPointProgress(...) (ACHRCuda.cu:361) that main launches
+ * once per output file (:741). The functions below are that seam, made
+ * callable: the CLI shipped with this library (ACHRCuda) is built on them,
+ * and any host language binds them with plain pointers and sizes.
+ *
+ * Conventions
+ *   - Every function returns ACHR_OK (0) or a negative ACHR_E_* code and
+ *     never exits the process (the reference exits or asserts:
+ *     ACHRcu/Utilities.cu:28-37, :76-85). achr_last_error() gives the text
+ *     of the last failure on the calling thread.
+ *   - Matrices are column major unless stated: the warmup set is n x W with
+ *     each point contiguous (ACHRCuda.cu:409, :712-713); the null-space basis
+ *     is n x d (ACHRCuda.cu:486-489). Sample output is reaction major,
+ *     out[i*P + chain], the layout of the reference's points buffer and of
+ *     its File<k> text files (ACHRCuda.cu:198-204, :744-756).
+ *   - Host pointers are caller-owned and only read during the call. Device
+ *     memory is owned by the handles. A handle is used by one host thread at
+ *     a time; distinct handles may run concurrently on distinct streams.
+ */
+#ifndef ACHR_H_
+#define ACHR_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACHR_OK 0
+#define ACHR_E_ARG (-1)      /* invalid argument or size */
+#define ACHR_E_IO (-2)       /* file missing or unreadable */
+#define ACHR_E_FORMAT (-3)   /* malformed MPS / CSV content */
+#define ACHR_E_DEVICE (-4)   /* HIP runtime or kernel launch failure */
+#define ACHR_E_SOLVER (-5)   /* rocSOLVER / null-space failure */
+#define ACHR_E_STATE (-6)    /* call out of order (e.g. no null space yet) */
+#define ACHR_E_NOMEM (-7)    /* host or device allocation failed */
+
+typedef struct achr_model achr_model;
+typedef struct achr_sampler achr_sampler;
+
+/* Text of the last error on this thread ("" if none). */
+const char *achr_last_error(void);
+/* Library version string. */
+const char *achr_version(void);
+
+/* ------------------------------------------------------------- model (a1) */
+
+/* Model from a sparse stoichiometric matrix S (m x n, coordinate format,
+ * duplicates summed, explicit zeros dropped) and reaction bounds.
+ * Replaces the dense-S + row-sorted COO assembly of ACHRCuda.cu:560-663;
+ * S is kept as CSR (columns ascending within a row) on host and device. */
+int achr_model_create(int n_rxns, int n_mets, int nnz, const int *rows, const int *cols,
+                      const double *vals, const double *lb, const double *ub, achr_model **out);
+
+/* Model from an MPS file in the subset the reference reads through
+ * glp_read_mps(GLP_MPS_DECK) (ACHRCuda.cu:560-597): N and E rows (the first N
+ * row is the objective and is not part of S), COLUMNS with one or two entries
+ * per line, RHS ignored, BOUNDS UP/LO/FX/FR/MI/PL (default bounds [0, +inf),
+ * an infinite bound is stored as +-DBL_MAX as GLPK reports it). Free-format
+ * whitespace splitting, so fixed-format decks read the same. */
+int achr_model_read_mps(const char *path, achr_model **out);
+
+void achr_model_destroy(achr_model *model);
+
+int achr_model_dims(const achr_model *model, int *n_rxns, int *n_mets, int *nnz, int *null_dim);
+
+/* Copies bounds out (n doubles each). */
+int achr_model_bounds(const achr_model *model, double *lb, double *ub);
+
+/* Copies the CSR form of S out: rowptr (m+1), colidx (nnz), vals (nnz). */
+int achr_model_csr(const achr_model *model, int *rowptr, int *colidx, double *vals);
+
+/* ------------------------------------------------------- null space (a3) */
+
+/* Null-space basis N of S computed on the GPU: eigenvectors of S^T S (FP64,
+ * rocSOLVER syevd) whose eigenvalue is below n * lambda_max * 2.2204e-16 —
+ * the singular-value rule of computeKernelCuda (ACHRCuda.cu:436-502, :474).
+ * Writes d to *null_dim. */
+int achr_model_compute_nullspace(achr_model *model, int *null_dim);
+
+/* Supplies N (n x d, column major) computed elsewhere. d may be 0. */
+int achr_model_set_nullspace(achr_model *model, const double *N, int d);
+
+/* Copies N out (n x d column major). */
+int achr_model_get_nullspace(const achr_model *model, double *N);
+
+/* -------------------------------------------------------- sampler (a2-a14) */
+
+typedef struct achr_options {
+  double dTol;          /* 1e-14, ACHRCuda.cu:512 */
+  double uTol;          /* 1e-9,  ACHRCuda.cu:508 */
+  double maxMinTol;     /* 1e-9,  ACHRCuda.cu:505 */
+  double reprojTol;     /* 1e-9,  ACHRCuda.cu:338 */
+  int check_every;      /* 10,    ACHRCuda.cu:325 */
+  int max_misses;       /* consecutive misses before a chain stops (reference: unbounded) */
+  int lanes;            /* threads per chain: 0 = autotuned; else 16..1024, power of two */
+  int reproject_mode;   /* 0 = auto, 1 = in-kernel per chain, 2 = batched GEMM rounds */
+  int trace_chains;     /* chains whose thinned trajectory is recorded (0 = none) */
+  int trace_thin;       /* record every trace_thin accepted steps */
+} achr_options;
+
+/* Fills the reference defaults. */
+void achr_options_default(achr_options *opts);
+
+/* Sampler over a model with a null space and a warmup set (n x W column
+ * major, each point contiguous). Computes the warmup centre on the host as
+ * (1/W) * sum_k w_k (createCenterPt, ACHRCuda.cu:414-434) and uploads the
+ * warmup set, bounds, S and N to device memory. opts may be NULL. */
+int achr_sampler_create(const achr_model *model, const double *warmup, int W, const achr_options *opts,
+                        achr_sampler **out);
+
+/* Same, reading the warmup CSV of the reference (one line per reaction, one
+ * comma-separated column per point; computenWrmup/parseLine,
+ * ACHRCuda.cu:387-412, :676-689). */
+int achr_sampler_create_csv(const achr_model *model, const char *warmup_csv, const achr_options *opts,
+                            achr_sampler **out);
+
+void achr_sampler_destroy(achr_sampler *sampler);
+
+/* Binds the sampler to a HIP stream (hipStream_t passed as void*; NULL = the
+ * null stream). */
+int achr_sampler_set_stream(achr_sampler *sampler, void *hip_stream);
+
+/* Centre of the warmup set (n doubles). */
+int achr_sampler_center(const achr_sampler *sampler, double *c0);
+
+/* Runs one output file: P chains, each started at the warmup centre, each
+ * making S accepted ACHR steps (stepPointProgress, ACHRCuda.cu:361-385 and
+ * the per-file loop :736-743). Uniforms come from Philox4x32-10 keyed by
+ * `seed` with counter {index, chain0 + chain, file, 0}. Results stay in
+ * device memory until fetched; the call returns once the work is queued.
+ */
+int achr_sample(achr_sampler *sampler, int P, int S, uint64_t seed, uint32_t file, int chain0);
+
+/* Waits for the last achr_sample and copies the points out: n x P reaction
+ * major (out[i*P + chain]). */
+int achr_sample_points(achr_sampler *sampler, double *out);
+
+/* Device pointer (as void*) of the n x P reaction-major points of the last
+ * run, valid until the next achr_sample or destroy. */
+int achr_sample_points_device(achr_sampler *sampler, void **dev_ptr);
+
+/* Per-chain counters of the last run, 4 int32 per chain: misses,
+ * reprojections, residual checks, flags (bit 0: stopped after max_misses). */
+int achr_sample_stats(achr_sampler *sampler, int32_t *stats);
+
+/* Trajectory of the first trace_chains chains, every trace_thin accepted
+ * steps: trace_chains x (S / trace_thin) x n doubles. */
+int achr_sample_trace(achr_sampler *sampler, double *trace);
+
+/* Milliseconds between HIP events recorded on the sampler's stream around
+ * the last run's kernels, split as: total, step kernels, projection GEMMs.
+ * Also the number of launches of each. */
+int achr_sample_timing(achr_sampler *sampler, double *ms_total, double *ms_step, double *ms_proj,
+                       int *n_step_launches, int *n_proj_launches);
+
+/* Schedule the autotuner picked for this sampler: threads per chain,
+ * reprojection mode (1 in-kernel, 2 batched). */
+int achr_sampler_schedule(const achr_sampler *sampler, int *lanes, int *reproject_mode);
+
+/* -------------------------------------------------------------- file I/O */
+
+/* Writes File<k>-style text: n lines of P comma-separated "%f" values
+ * (ACHRCuda.cu:744-756). */
+int achr_write_points_csv(const char *path, const double *points, int n, int P);
+
+/* Reads a warmup CSV into a newly malloc'ed n x W column-major array
+ * (release with achr_free). */
+int achr_read_warmup_csv(const char *path, int n_expected, double **warmup, int *W);
+
+void achr_free(void *p);
+
+/* --------------------------------------------------- test/debug helpers */
+
+/* Philox4x32-10 block of the library's stream (host implementation of the
+ * same header the kernels use). */
+void achr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACHR_H_ */

@@ -1,0 +1,128 @@
+"""Host-side boundary: the C ABI loads and exports what include/achr.h declares,
+and the MPS / CSV readers and writers (host code, no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import achr
+from achr import _lib
+from conftest import ROOT, golden, parse_mps_py
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "achr.h")).read()
+    declared = set(re.findall(r"\b(achr_[a-z0-9_]+)\s*\(", hdr))
+    assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+    L = achr.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert b"gfx950" in L.achr_version()
+
+
+def test_options_defaults_are_the_reference_tolerances():
+    o = _lib.AchrOptions()
+    achr.lib().achr_options_default(ctypes.byref(o))
+    assert (o.dTol, o.uTol, o.maxMinTol, o.reprojTol, o.check_every) == (1e-14, 1e-9, 1e-9, 1e-9, 10)
+
+
+def test_library_philox_matches_kat():
+    assert achr.philox4x32_10((0, 0, 0, 0), (0, 0)) == (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)
+    rows = [l.split() for l in open(golden("philox_rocrand.txt")) if not l.startswith("#")]
+    for r in rows[:16]:
+        v = [int(t, 16) for t in r]
+        assert achr.philox4x32_10(tuple(v[0:4]), tuple(v[4:6])) == tuple(v[6:10])
+
+
+def test_read_mps_p_putida_counts():
+    """The model the reference ships (VFWarmup/lib/P_Putida.mps): 911 E rows +
+    the objective, 1,060 columns; 1,040 upper bounds at 1000 and 20 at 0; lower
+    bounds 347 at -1000, 11 at -100, one at -20, one at -6.3, the rest 0."""
+    m = achr.Model.read_mps(golden("P_Putida.mps"))
+    n, mm, nnz, d = m.dims
+    assert (n, mm, d) == (1060, 911, -1)
+    lb, ub = m.bounds()
+    assert (ub == 1000).sum() == 1040 and (ub == 0).sum() == 20
+    assert (lb == -1000).sum() == 347 and (lb == -100).sum() == 11
+    assert (lb == -20).sum() == 1 and (lb == -6.3).sum() == 1 and (lb == 0).sum() == 700
+    S, L, U = parse_mps_py(golden("P_Putida.mps"))
+    np.testing.assert_array_equal(m.dense(), S)
+    np.testing.assert_array_equal(lb, L)
+    np.testing.assert_array_equal(ub, U)
+    assert nnz == np.count_nonzero(S)
+
+
+def test_read_mps_c1_matches_python_parse(c1):
+    S, lb, ub, _, _ = c1
+    m = achr.Model.read_mps(golden("c1.mps"))
+    np.testing.assert_array_equal(m.dense(), S)
+    L, U = m.bounds()
+    np.testing.assert_array_equal(L, lb)
+    np.testing.assert_array_equal(U, ub)
+    assert (lb == ub).sum() == 5
+
+
+def test_mps_bound_types_and_defaults(tmp_path):
+    p = tmp_path / "b.mps"
+    p.write_text("NAME x\nROWS\n N  obj\n E  r1\n N  free2\nCOLUMNS\n    a  r1  1  obj  3\n    b  r1  -2\n"
+                 "    c  free2  1\n    d  r1  1\nRHS\n    rhs  r1  5\nBOUNDS\n UP BND a -4\n FR BND b\n MI BND c\n"
+                 " FX BND d 2.5\nENDATA\n")
+    m = achr.Model.read_mps(str(p))
+    lb, ub = m.bounds()
+    big = np.finfo(float).max
+    np.testing.assert_array_equal(lb, [-big, -big, -big, 2.5])
+    np.testing.assert_array_equal(ub, [-4, big, big, 2.5])
+    # first N row is the objective; a later N row stays a (free) row of S; RHS is ignored
+    np.testing.assert_array_equal(m.dense(), [[1, -2, 0, 1], [0, 0, 1, 0]])
+
+
+@pytest.mark.parametrize("text,code", [
+    ("ROWS\n E r\nCOLUMNS\n    a  q  1\nENDATA\n", -3),       # unknown row
+    ("ROWS\n E r\nCOLUMNS\n    a  r  x1\nENDATA\n", -3),      # bad number
+    ("ROWS\n E r\nCOLUMNS\n    a  r  1\n", -3),               # no ENDATA
+])
+def test_mps_errors(tmp_path, text, code):
+    p = tmp_path / "bad.mps"
+    p.write_text(text)
+    with pytest.raises(achr.AchrError) as e:
+        achr.Model.read_mps(str(p))
+    assert e.value.code == code
+    with pytest.raises(achr.AchrError) as e:
+        achr.Model.read_mps(str(tmp_path / "missing.mps"))
+    assert e.value.code == -2
+
+
+def test_warmup_csv_reader(c1):
+    _, _, _, Wf, We = c1
+    W = achr.read_warmup_csv(golden("c1_warmup_exact.csv"), 95)
+    np.testing.assert_array_equal(W, We)
+    W = achr.read_warmup_csv(golden("c1_warmup.csv"), 95)
+    np.testing.assert_array_equal(W, Wf)
+
+
+def test_warmup_csv_errors(tmp_path):
+    p = tmp_path / "w.csv"
+    p.write_text("1,2,3\n4,5\n")
+    with pytest.raises(achr.AchrError) as e:
+        achr.read_warmup_csv(str(p), 2)
+    assert e.value.code == -3
+    p.write_text("1,2\n3,4\n")
+    with pytest.raises(achr.AchrError):
+        achr.read_warmup_csv(str(p), 3)  # reaction count mismatch
+
+
+def test_points_csv_writer_layout(tmp_path):
+    pts = np.array([[1.0, -2.5, 3.1234567], [0.0, 1e-7, -0.0000004]])
+    p = tmp_path / "File0"
+    achr.write_points_csv(str(p), pts)
+    lines = p.read_text().splitlines()
+    assert lines == ["1.000000,-2.500000,3.123457", "0.000000,0.000000,-0.000000"]
+
+
+def test_model_create_validates():
+    with pytest.raises(achr.AchrError):
+        achr.Model.from_coo(3, 2, [0, 5], [0, 1], [1.0, 1.0], np.zeros(3), np.ones(3))
+    m = achr.Model.from_coo(3, 2, [0, 0, 1, 0], [0, 1, 2, 0], [1.0, 2.0, 3.0, 1.0], np.zeros(3), np.ones(3))
+    np.testing.assert_array_equal(m.dense(), [[2, 2, 0], [0, 0, 3]])  # duplicates summed

@@ -1,0 +1,163 @@
+"""GPU parity: the HIP sampler, called through the C ABI, against the CPU oracle
+on the same inputs and Philox stream. Integer/step structure is compared
+exactly and the FP64 trajectories bit for bit (the kernels and the oracle
+share one arithmetic contract: the same fused multiply-adds and the same
+reduction tree for the chain's lane count)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import achr
+import oracle as O
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1_setup(c1):
+    S, lb, ub, Wf, We = c1
+    N = O.nullspace(S)
+    rp, ci, va = O.csr_of(S)
+    model = achr.Model.from_dense(S, lb, ub)
+    model.set_nullspace(N)
+    return dict(S=S, lb=lb, ub=ub, Wf=Wf, We=We, N=N, rp=rp, ci=ci, va=va, model=model)
+
+
+def run_both(st, warm, *, lanes, mode, P, S, seed, file=0, chain0=0, trace=0, thin=0, threads=8):
+    say(f"case lanes={lanes} mode={mode} P={P} S={S} seed={seed} file={file} chain0={chain0}")
+    opts = achr.Options(lanes=lanes, reproject_mode=mode, trace_chains=trace, trace_thin=thin)
+    smp = achr.Sampler(st["model"], warm, opts)
+    assert smp.schedule() == (lanes, mode)
+    smp.run(P, S, seed=seed, file=file, chain0=chain0)
+    g_pts, g_st = smp.points(), smp.stats()
+    g_tr = smp.trace() if trace else None
+    o_pts, o_st, o_tr = O.sample(warm=warm, lb=st["lb"], ub=st["ub"], rowptr=st["rp"], colidx=st["ci"],
+                                 vals=st["va"], N=st["N"], m=st["S"].shape[0], P=P, S=S, seed=seed, file=file,
+                                 chain0=chain0, lanes=lanes, threads=threads, trace_chains=trace, trace_thin=thin)
+    return g_pts, g_st, g_tr, o_pts, o_st, o_tr
+
+
+@pytest.mark.parametrize("lanes", [16, 32, 64, 128, 256, 1024])
+@pytest.mark.parametrize("which", ["csv6", "exact"])
+def test_in_kernel_projection_bitwise(c1_setup, lanes, which):
+    warm = c1_setup["Wf"] if which == "csv6" else c1_setup["We"]
+    g, gs, _, o, os_, _ = run_both(c1_setup, warm, lanes=lanes, mode=1, P=130, S=250, seed=2024)
+    np.testing.assert_array_equal(gs, os_)
+    np.testing.assert_array_equal(g, o)
+    if which == "csv6":
+        assert (gs[:, 1] == 25).all()  # every check reprojects
+    else:
+        assert (gs[:, 1] == 0).all()
+
+
+@pytest.mark.parametrize("lanes", [32, 256])
+def test_batched_projection_bitwise(c1_setup, lanes):
+    g, gs, _, o, os_, _ = run_both(c1_setup, c1_setup["Wf"], lanes=lanes, mode=2, P=200, S=123, seed=5)
+    np.testing.assert_array_equal(gs, os_)
+    np.testing.assert_array_equal(g, o)
+
+
+def test_trace_file_and_chain_offset(c1_setup):
+    g, gs, gt, o, os_, ot = run_both(c1_setup, c1_setup["Wf"], lanes=64, mode=1, P=40, S=100, seed=9, file=3,
+                                     chain0=1000, trace=5, thin=10)
+    np.testing.assert_array_equal(g, o)
+    np.testing.assert_array_equal(gt, ot[:5, :10])
+    assert np.all(g >= c1_setup["lb"][:, None]) and np.all(g <= c1_setup["ub"][:, None])
+
+
+def test_medium_model_batched_gemm_bitwise():
+    """A 330 x 600 model (d ~ 270): the batched FP64 GEMM path with chains
+    flagged every round, against the oracle's sequential dot products."""
+    from achr import synth
+    import scipy.linalg
+    m = synth.make_model(330, 600, 17, irrev_frac=0.3, n_fixed=6, width=(5.0, 50.0))
+    N = scipy.linalg.null_space(m.dense())
+    W = synth.round_like_csv(synth.chord_warmup(N, m.x0, m.lb, m.ub, 400, seed=3))
+    model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
+    model.set_nullspace(N)
+    rp, ci, va = O.csr_of(m.dense())
+    st = dict(S=m.dense(), lb=m.lb, ub=m.ub, N=N, rp=rp, ci=ci, va=va, model=model)
+    for lanes in (64, 256):
+        g, gs, _, o, os_, _ = run_both(st, W, lanes=lanes, mode=2, P=96, S=60, seed=31)
+        np.testing.assert_array_equal(gs, os_)
+        assert gs[:, 1].sum() > 0
+        np.testing.assert_array_equal(g, o)
+
+
+def test_gpu_nullspace_matches_scipy(c1_setup):
+    import scipy.linalg
+    m = achr.Model.read_mps(golden("c1.mps"))
+    d = m.compute_nullspace()
+    N = m.nullspace()
+    N2 = scipy.linalg.null_space(c1_setup["S"])
+    assert d == N2.shape[1] == 23
+    assert np.abs(c1_setup["S"] @ N).max() < 1e-12
+    assert np.abs(N.T @ N - np.eye(d)).max() < 1e-12
+    assert np.abs(N @ N.T - N2 @ N2.T).max() < 1e-12
+
+
+def test_gpu_nullspace_rank_deficient():
+    rng = np.random.default_rng(0)
+    A = rng.integers(-2, 3, size=(20, 60)).astype(float)
+    S = np.vstack([A, A[0] + A[1], A[2] - A[3]])
+    m = achr.Model.from_dense(S, -np.ones(60), np.ones(60))
+    d = m.compute_nullspace()
+    N = m.nullspace()
+    assert d == 60 - np.linalg.matrix_rank(S) == 40
+    assert np.abs(S @ N).max() < 1e-10
+
+
+def test_box_uniform_moments_gpu():
+    lb = np.array([0.0, -1.0, 2.0, -5.0])
+    ub = np.array([1.0, 3.0, 2.5, 5.0])
+    rng = np.random.default_rng(5)
+    Wm = lb[:, None] + (ub - lb)[:, None] * rng.uniform(0, 1, (4, 20))
+    model = achr.Model.from_coo(4, 0, [], [], [], lb, ub)
+    model.set_nullspace(np.eye(4))
+    pts = achr.Sampler(model, Wm, achr.Options(lanes=16)).sample(20000, 60, seed=11)
+    np.testing.assert_allclose(pts.mean(axis=1), (lb + ub) / 2, atol=0.01 * (ub - lb).max())
+    np.testing.assert_allclose(pts.std(axis=1), (ub - lb) / np.sqrt(12), rtol=0.02)
+
+
+def test_miss_cap_forced_from_host():
+    """Warmup points all equal to their centre: every direction is zero; each
+    chain stops after max_misses and is flagged (the reference would spin)."""
+    lb, ub = -np.ones(3), np.ones(3)
+    model = achr.Model.from_coo(3, 0, [], [], [], lb, ub)
+    model.set_nullspace(np.eye(3))
+    smp = achr.Sampler(model, np.zeros((3, 4)), achr.Options(lanes=16, max_misses=50))
+    pts = smp.sample(8, 5, seed=0)
+    st = smp.stats()
+    assert (st[:, 3] == 1).all() and (st[:, 0] == 51).all()
+    np.testing.assert_array_equal(pts, 0.0)
+
+
+def test_cli_drop_in(tmp_path, c1_setup):
+    """ACHRCuda model.mps warmup.csv nFiles nPoints nSteps writes File0..: n
+    lines x P comma-separated %f values, equal to the oracle's points."""
+    npath = tmp_path / "N.bin"
+    np.asfortranarray(c1_setup["N"]).ravel(order="F").tofile(npath)
+    env = dict(os.environ, ACHR_SEED="77", ACHR_NULLSPACE=str(npath))
+    say("cli run")
+    r = subprocess.run([achr.CLI_PATH, golden("c1.mps"), golden("c1_warmup.csv"), "2", "50", "80"],
+                       cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for f in range(2):
+        lines = (tmp_path / f"File{f}").read_text().splitlines()
+        assert len(lines) == 95 and all(len(l.split(",")) == 50 for l in lines)
+        o, _, _ = O.sample(warm=c1_setup["Wf"], lb=c1_setup["lb"], ub=c1_setup["ub"], rowptr=c1_setup["rp"],
+                           colidx=c1_setup["ci"], vals=c1_setup["va"], N=c1_setup["N"], m=72, P=50, S=80, seed=77,
+                           file=f, lanes=32)
+        expect = [",".join("%f" % v for v in o[i]) for i in range(95)]
+        assert lines == expect
+    r = subprocess.run([achr.CLI_PATH, "missing.mps", "w.csv", "1", "1", "1"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode != 0 and "MPS" in r.stderr
