@@ -152,10 +152,22 @@ def fva_warmup(model: SynthModel) -> np.ndarray:
     return move_to_center(W)
 
 
+def free_directions(N: np.ndarray, lb: np.ndarray, ub: np.ndarray) -> np.ndarray:
+    """Orthonormal basis of {v in span(N) : v_i = 0 for fixed reactions (lb = ub)}."""
+    import scipy.linalg
+    fixed = np.nonzero(lb == ub)[0]
+    if len(fixed) == 0:
+        return N
+    Z = scipy.linalg.null_space(N[fixed, :])
+    return N @ Z
+
+
 def chord_warmup(N: np.ndarray, x0: np.ndarray, lb: np.ndarray, ub: np.ndarray, W: int, seed: int,
                  shrink: float = 0.999) -> np.ndarray:
-    """W points on random null-space chords through x0, then moved to the centre."""
+    """W points on random chords through x0 along directions that keep S x = 0
+    and the fixed reactions fixed, then moved to the centre."""
     rng = np.random.default_rng(seed)
+    N = free_directions(N, lb, ub)
     n, d = N.shape
     out = np.empty((n, W))
     B = 512

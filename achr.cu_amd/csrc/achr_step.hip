@@ -193,7 +193,7 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
   constexpr bool kCL = E > 8;
   double x[E];
   double c[kCL ? 1 : E];
-  double* cl = lds + 4 * kRedSlots;
+  double* cl = lds + 4 * kRedSlots + static_cast<size_t>(g) * (TPC * E);  // this chain's slice
   if (st.z == kFresh) {
     const rsrc_t r_c0 = make_rsrc(a.c0, 8LL * n);
 #pragma unroll
@@ -385,6 +385,12 @@ template <int TPC, int E>
 hipError_t launch_tpc_e(const StepArgs& a, hipStream_t stream, size_t lds) {
   constexpr int CPB = TPC <= 64 ? 256 / TPC : 1;
   const unsigned grid = static_cast<unsigned>((a.P + CPB - 1) / CPB);
+  if (lds > 65536) {
+    // Dynamic LDS beyond 64 KiB must be opted into per kernel.
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&step_kernel<TPC, E, CPB>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL((step_kernel<TPC, E, CPB>), dim3(grid), dim3(TPC * CPB), lds, stream, a);
   return hipGetLastError();
 }
@@ -422,7 +428,7 @@ int elems_for(int n, int tpc) {
 
 size_t step_lds_bytes(const StepArgs&, const Schedule& s) {
   size_t bytes = 4 * kRedSlots * sizeof(double);
-  if (s.e > 8) bytes += static_cast<size_t>(s.tpc) * s.e * sizeof(double);  // centre in LDS
+  if (s.e > 8) bytes += static_cast<size_t>(s.cpb) * s.tpc * s.e * sizeof(double);  // centres in LDS
   return bytes;
 }
 
