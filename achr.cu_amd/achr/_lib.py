@@ -61,6 +61,8 @@ SIGNATURES = {
     "achr_read_warmup_csv": (_I, [C.c_char_p, _I, C.POINTER(_PD), _PI]),
     "achr_free": (None, [_P]),
     "achr_philox4x32_10": (None, [_PU32, _PU32, _PU32]),
+    "achr_gemm_f32": (_I, [_I, _I, _I, C.POINTER(C.c_float), C.c_long, C.c_long, C.POINTER(C.c_float), C.c_long,
+                           C.c_long, C.POINTER(C.c_float), C.c_long, C.c_long, C.c_float, C.c_float, _I]),
 }
 
 
@@ -78,6 +80,8 @@ class AchrOptions(C.Structure):
         ("reproject_mode", C.c_int),
         ("trace_chains", C.c_int),
         ("trace_thin", C.c_int),
+        ("proj_prec", C.c_int),
+        ("fp64_every", C.c_int),
     ]
 
 

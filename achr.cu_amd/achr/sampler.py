@@ -7,6 +7,7 @@ run files of P chains x S steps. Arrays cross the boundary as numpy arrays.
 from __future__ import annotations
 
 import ctypes as C
+import ctypes as _ctypes
 from dataclasses import dataclass
 
 import numpy as np
@@ -131,10 +132,16 @@ class Options:
     reproject_mode: int = 0
     trace_chains: int = 0
     trace_thin: int = 0
+    proj_prec: int = 0
+    fp64_every: int = 10
 
     def to_c(self) -> AchrOptions:
         return AchrOptions(self.dTol, self.uTol, self.maxMinTol, self.reprojTol, self.check_every,
-                           self.max_misses, self.lanes, self.reproject_mode, self.trace_chains, self.trace_thin)
+                           self.max_misses, self.lanes, self.reproject_mode, self.trace_chains, self.trace_thin,
+                           self.proj_prec, self.fp64_every)
+
+
+PROJ_FP64, PROJ_RES_BF16X3, PROJ_RES_F32, PROJ_NULL_F32, PROJ_NULL_BF16X3 = range(5)
 
 
 class Sampler:
@@ -242,3 +249,30 @@ def read_warmup_csv(path: str, n_expected: int = 0) -> np.ndarray:
         return flat.reshape((n_expected, W.value), order="F").copy()
     finally:
         lib().achr_free(ptr)
+
+
+def gemm_f32(A, B, prec: int = 0, alpha: float = 1.0, C=None, beta: float = 0.0, transA=False, transB=False):
+    """C = alpha * op(A) op(B) + beta * C on the matrix cores (prec 0 FP32, 1 BF16x3, 2 BF16).
+
+    op(X) = X.T when trans is set; the transpose is a stride swap, not a copy."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    if transA:
+        K, M = A.shape
+        sa_m, sa_k = 1, M
+    else:
+        M, K = A.shape
+        sa_m, sa_k = K, 1
+    if transB:
+        N, K2 = B.shape
+        sb_k, sb_n = 1, K2
+    else:
+        K2, N = B.shape
+        sb_k, sb_n = N, 1
+    if K2 != K:
+        raise ValueError("inner dimensions differ")
+    out = np.zeros((M, N), dtype=np.float32) if C is None else np.array(C, dtype=np.float32, order="C")
+    P = _ctypes.POINTER(_ctypes.c_float)
+    check(lib().achr_gemm_f32(M, N, K, A.ctypes.data_as(P), sa_m, sa_k, B.ctypes.data_as(P), sb_k, sb_n,
+                              out.ctypes.data_as(P), N, 1, alpha, beta, prec), "achr_gemm_f32")
+    return out

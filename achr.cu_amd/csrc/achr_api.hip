@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/achr.h"
+#include "achr_gemm.h"
 #include "achr_internal.h"
 #include "achr_philox.h"
 
@@ -90,6 +91,11 @@ struct achr_sampler {
   std::vector<double> c0;
   DevBuf<double> warm, lb, ub, c0d, vals, Ncm, Nrm;
   DevBuf<int> rowptr, colidx;
+  // reduced-precision projection operands (FP32 copies)
+  DevBuf<float> Sp32;   // pseudo-inverse S+, n x m column major (i contiguous)
+  DevBuf<float> N32;    // N, n x d column major
+  DevBuf<float> R32, T32;  // per-chain residuals (P x m) and N^T x (P x d)
+  int rounds_fp64 = 0, rounds_low = 0;
   // run state
   int P = 0, S = 0;
   DevBuf<double> X, C, T, out, trace;
@@ -130,6 +136,8 @@ void achr_options_default(achr_options* o) {
   o->reproject_mode = 0;
   o->trace_chains = 0;
   o->trace_thin = 0;
+  o->proj_prec = ACHR_PROJ_FP64;
+  o->fp64_every = 10;
 }
 
 // ------------------------------------------------------------------ model
@@ -439,6 +447,113 @@ int achr_model_compute_nullspace(achr_model* M, int* null_dim) {
   return ACHR_OK;
 }
 
+// ------------------------------------------------------ pseudo-inverse S+
+
+// S+ = S^T (S S^T)+ with the eigen-decomposition of S S^T (rocSOLVER syevd,
+// eigenvalues below m * lambda_max * 2.2204e-16 dropped, the rule the
+// reference applies to S^T S) and one rocBLAS DGEMM; stored as FP32.
+// x - S+ (S x) is the orthogonal projection of x onto null(S), the same
+// operator as N N^T x.
+static int build_pinv_f32(const achr_model* M, DevBuf<float>& out) {
+  const int n = M->n, m = M->m;
+  if (m == 0) {
+    HIP_TRY(out.ensure(1));
+    return ACHR_OK;
+  }
+  // S S^T on the host from the columns of S.
+  std::vector<std::vector<std::pair<int, double>>> colent(static_cast<size_t>(n));
+  for (int r = 0; r < m; ++r)
+    for (int p = M->rowptr[r]; p < M->rowptr[r + 1]; ++p) colent[M->colidx[p]].emplace_back(r, M->vals[p]);
+  std::vector<double> G(static_cast<size_t>(m) * m, 0.0);
+  for (int j = 0; j < n; ++j)
+    for (const auto& a : colent[j])
+      for (const auto& b : colent[j]) G[static_cast<size_t>(a.first) + static_cast<size_t>(b.first) * m] += a.second * b.second;
+  std::vector<double> Sd(static_cast<size_t>(m) * n, 0.0);  // dense S, m x n column major
+  for (int r = 0; r < m; ++r)
+    for (int p = M->rowptr[r]; p < M->rowptr[r + 1]; ++p)
+      Sd[static_cast<size_t>(r) + static_cast<size_t>(M->colidx[p]) * m] = M->vals[p];
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return fail(ACHR_E_SOLVER, "rocblas_create_handle");
+  double *dG = nullptr, *dD = nullptr, *dE = nullptr, *dW = nullptr, *dS = nullptr, *dP = nullptr;
+  rocblas_int* dinfo = nullptr;
+  int rc = ACHR_OK;
+  std::vector<double> ev(m);
+  do {
+    const size_t mm = static_cast<size_t>(m) * m, mn = static_cast<size_t>(m) * n;
+    if (hipMalloc(&dG, 8 * mm) != hipSuccess || hipMalloc(&dD, 8 * static_cast<size_t>(m)) != hipSuccess ||
+        hipMalloc(&dE, 8 * static_cast<size_t>(m)) != hipSuccess || hipMalloc(&dinfo, sizeof(rocblas_int)) != hipSuccess ||
+        hipMalloc(&dW, 8 * mm) != hipSuccess || hipMalloc(&dS, 8 * mn) != hipSuccess ||
+        hipMalloc(&dP, 8 * mn) != hipSuccess) {
+      rc = fail(ACHR_E_NOMEM, "device allocation for the pseudo-inverse");
+      break;
+    }
+    if (hipMemcpy(dG, G.data(), 8 * mm, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dS, Sd.data(), 8 * mn, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = fail(ACHR_E_DEVICE, "upload S S^T / S");
+      break;
+    }
+    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, m, dG, m, dD, dE, dinfo) !=
+        rocblas_status_success) {
+      rc = fail(ACHR_E_SOLVER, "rocsolver_dsyevd (S S^T)");
+      break;
+    }
+    rocblas_int info = 0;
+    if (hipMemcpy(&info, dinfo, sizeof info, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ev.data(), dD, 8 * static_cast<size_t>(m), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(G.data(), dG, 8 * mm, hipMemcpyDeviceToHost) != hipSuccess || info != 0) {
+      rc = fail(ACHR_E_SOLVER, "syevd (S S^T) info %d", static_cast<int>(info));
+      break;
+    }
+    double lmax = 0.0;
+    for (double v : ev) lmax = std::max(lmax, std::fabs(v));
+    const double tol = static_cast<double>(m) * lmax * 2.2204e-16;
+    // U diag(1/lambda) for kept eigenpairs, zero columns otherwise
+    std::vector<double> Us(G.size());
+    for (int k = 0; k < m; ++k) {
+      const double sc = ev[k] >= tol ? 1.0 / ev[k] : 0.0;
+      for (int i = 0; i < m; ++i) Us[static_cast<size_t>(i) + static_cast<size_t>(k) * m] = G[static_cast<size_t>(i) + static_cast<size_t>(k) * m] * sc;
+    }
+    if (hipMemcpy(dW, Us.data(), 8 * mm, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = fail(ACHR_E_DEVICE, "upload U/lambda");
+      break;
+    }
+    const double one = 1.0, zero = 0.0;
+    // dE is reused only as scratch below; Ginv = (U/lambda) U^T into dG's slot needs U itself: dG holds U.
+    double* dGinv = nullptr;
+    if (hipMalloc(&dGinv, 8 * mm) != hipSuccess) {
+      rc = fail(ACHR_E_NOMEM, "device allocation (S S^T)+");
+      break;
+    }
+    rocblas_status st1 = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, m, m, &one, dW, m,
+                                       dG, m, &zero, dGinv, m);
+    // S+ = S^T Ginv : (n x m) = (m x n)^T (m x m)
+    rocblas_status st2 = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, m, m, &one, dS, m,
+                                       dGinv, m, &zero, dP, n);
+    (void)hipFree(dGinv);
+    if (st1 != rocblas_status_success || st2 != rocblas_status_success) {
+      rc = fail(ACHR_E_SOLVER, "rocblas_dgemm for S+");
+      break;
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+      rc = fail(ACHR_E_DEVICE, "pseudo-inverse");
+      break;
+    }
+    hipError_t e = out.ensure(mn);
+    if (e == hipSuccess) e = achr::launch_to_f32(dP, out.p, mn, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = fail(ACHR_E_DEVICE, "S+ to FP32: %s", hipGetErrorString(e));
+  } while (false);
+  (void)hipFree(dG);
+  (void)hipFree(dD);
+  (void)hipFree(dE);
+  (void)hipFree(dinfo);
+  (void)hipFree(dW);
+  (void)hipFree(dS);
+  (void)hipFree(dP);
+  rocblas_destroy_handle(h);
+  return rc;
+}
+
 // ---------------------------------------------------------------- sampler
 
 static achr::Schedule pick_schedule(int n, int lanes_req) {
@@ -483,6 +598,10 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
   if (o.lanes != 0 && !achr::schedule_supported(o.lanes, achr::elems_for(M->n, o.lanes)))
     return fail(ACHR_E_ARG, "lanes=%d cannot hold n=%d reactions", o.lanes, M->n);
   if (o.trace_chains > 0 && o.trace_thin <= 0) return fail(ACHR_E_ARG, "trace_thin must be > 0");
+  if (o.proj_prec < ACHR_PROJ_FP64 || o.proj_prec > ACHR_PROJ_NULL_BF16X3 || o.fp64_every < 1)
+    return fail(ACHR_E_ARG, "bad proj_prec %d / fp64_every %d", o.proj_prec, o.fp64_every);
+  if (o.proj_prec != ACHR_PROJ_FP64 && o.reproject_mode == 1)
+    return fail(ACHR_E_ARG, "reduced-precision projection runs in batched rounds (reproject_mode 2)");
   auto* S = new (std::nothrow) achr_sampler();
   if (!S) return fail(ACHR_E_NOMEM, "sampler allocation");
   S->n = M->n;
@@ -505,7 +624,7 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
     return fail(ACHR_E_ARG, "n=%d exceeds the largest register schedule (1024 x 16)", M->n);
   }
   const double nd = static_cast<double>(M->n) * static_cast<double>(M->d);
-  S->mode = o.reproject_mode != 0 ? o.reproject_mode : (nd <= 65536.0 ? 1 : 2);
+  S->mode = o.reproject_mode != 0 ? o.reproject_mode : (nd <= 65536.0 && o.proj_prec == ACHR_PROJ_FP64 ? 1 : 2);
   // Warmup centre on the host: (1/W) * sum_k w_k, k ascending (createCenterPt).
   const int n = M->n;
   S->c0.assign(n, 0.0);
@@ -529,11 +648,23 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
   if (e == hipSuccess) e = S->vals.upload(M->vals.data(), M->vals.size(), s0);
   if (e == hipSuccess) e = S->Ncm.upload(M->N.data(), M->N.size(), s0);
   if (e == hipSuccess) e = S->Nrm.upload(Nrm.data(), Nrm.size(), s0);
+  if (e == hipSuccess && (o.proj_prec == ACHR_PROJ_NULL_F32 || o.proj_prec == ACHR_PROJ_NULL_BF16X3)) {
+    e = S->N32.ensure(std::max<size_t>(M->N.size(), 1));
+    if (e == hipSuccess) e = achr::launch_to_f32(S->Ncm.p, S->N32.p, M->N.size(), nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+  }
   if (e == hipSuccess) e = hipEventCreate(&S->ev_begin);
   if (e == hipSuccess) e = hipEventCreate(&S->ev_end);
   if (e != hipSuccess) {
     achr_sampler_destroy(S);
     return fail(ACHR_E_DEVICE, "sampler upload: %s", hipGetErrorString(e));
+  }
+  if (o.proj_prec == ACHR_PROJ_RES_BF16X3 || o.proj_prec == ACHR_PROJ_RES_F32) {
+    const int rc = build_pinv_f32(M, S->Sp32);
+    if (rc != ACHR_OK) {
+      achr_sampler_destroy(S);
+      return rc;
+    }
   }
   *out = S;
   return ACHR_OK;
@@ -660,6 +791,13 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   HIP_TRY(S->flag.ensure(P));
   HIP_TRY(S->idx.ensure(P));
   HIP_TRY(S->count.ensure(1));
+  const int prec = S->opt.proj_prec;
+  const bool res_form = prec == ACHR_PROJ_RES_BF16X3 || prec == ACHR_PROJ_RES_F32;
+  const bool null_low = prec == ACHR_PROJ_NULL_F32 || prec == ACHR_PROJ_NULL_BF16X3;
+  if (res_form) HIP_TRY(S->R32.ensure(static_cast<size_t>(P) * std::max(S->m, 1)));
+  if (null_low) HIP_TRY(S->T32.ensure(static_cast<size_t>(P) * std::max(S->d, 1)));
+  S->rounds_fp64 = 0;
+  S->rounds_low = 0;
   if (tr_chains > 0 && tr_len > 0) HIP_TRY(S->trace.ensure(static_cast<size_t>(tr_chains) * tr_len * n));
   S->P = P;
   S->S = Ssteps;
@@ -702,6 +840,7 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   a.st = S->st.p;
   a.stats = S->stats.p;
   a.flag = S->flag.p;
+  a.R32 = res_form ? S->R32.p : nullptr;
   a.out = S->out.p;
   a.trace = S->trace_len > 0 ? S->trace.p : nullptr;
   a.trace_chains = tr_chains;
@@ -731,7 +870,75 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
       if (rc != ACHR_OK) return rc;
       HIP_TRY(hipEventRecord(e0, st));
       le = achr::launch_compact(S->flag.p, P, S->idx.p, S->count.p, st);
-      if (le == hipSuccess) le = achr::launch_project(S->Ncm.p, n, S->d, S->X.p, S->T.p, S->idx.p, S->count.p, P, st);
+      const bool fp64_round = prec == ACHR_PROJ_FP64 || (seg + 1) % S->opt.fp64_every == 0;
+      if (le == hipSuccess && fp64_round) {
+        le = achr::launch_project(S->Ncm.p, n, S->d, S->X.p, S->T.p, S->idx.p, S->count.p, P, st);
+        ++S->rounds_fp64;
+      } else if (le == hipSuccess && res_form) {
+        // X(q, :) -= sum_k R(q, k) S+(:, k): A = residual rows, B(k, i) = S+[i + k*n]
+        achr::GemmArgs g{};
+        g.M = P;
+        g.N = n;
+        g.K = S->m;
+        g.Mdev = S->count.p;
+        g.A = S->R32.p;
+        g.sa_m = S->m;
+        g.sa_k = 1;
+        g.idxA = S->idx.p;
+        g.B = S->Sp32.p;
+        g.sb_k = n;
+        g.sb_n = 1;
+        g.C = S->X.p;
+        g.sc_m = n;
+        g.sc_n = 1;
+        g.idxC = S->idx.p;
+        g.alpha = 1.0f;
+        g.epi = achr::kEpiSubF64;
+        le = achr::launch_gemm(g, prec == ACHR_PROJ_RES_F32 ? achr::kPrecF32 : achr::kPrecBF16x3, P, st);
+        ++S->rounds_low;
+      } else if (le == hipSuccess) {
+        const int gp = prec == ACHR_PROJ_NULL_F32 ? achr::kPrecF32 : achr::kPrecBF16x3;
+        // T(q, j) = sum_i X(q, i) N(i, j)
+        achr::GemmArgs g{};
+        g.M = P;
+        g.N = S->d;
+        g.K = n;
+        g.Mdev = S->count.p;
+        g.A = S->X.p;
+        g.a_f64 = 1;
+        g.sa_m = n;
+        g.sa_k = 1;
+        g.idxA = S->idx.p;
+        g.B = S->N32.p;
+        g.sb_k = 1;
+        g.sb_n = n;
+        g.C = S->T32.p;
+        g.sc_m = S->d;
+        g.sc_n = 1;
+        g.alpha = 1.0f;
+        g.epi = achr::kEpiStoreF32;
+        le = achr::launch_gemm(g, gp, P, st);
+        // X(q, i) = sum_j T(q, j) N(i, j)
+        achr::GemmArgs h{};
+        h.M = P;
+        h.N = n;
+        h.K = S->d;
+        h.Mdev = S->count.p;
+        h.A = S->T32.p;
+        h.sa_m = S->d;
+        h.sa_k = 1;
+        h.B = S->N32.p;
+        h.sb_k = n;
+        h.sb_n = 1;
+        h.C = S->X.p;
+        h.sc_m = n;
+        h.sc_n = 1;
+        h.idxC = S->idx.p;
+        h.alpha = 1.0f;
+        h.epi = achr::kEpiStoreF64;
+        if (le == hipSuccess) le = achr::launch_gemm(h, gp, P, st);
+        ++S->rounds_low;
+      }
       if (le != hipSuccess) return fail(ACHR_E_DEVICE, "projection launch: %s", hipGetErrorString(le));
       HIP_TRY(hipEventRecord(e1, st));
       S->ev_proj.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
@@ -798,6 +1005,41 @@ int achr_sample_timing(achr_sampler* S, double* ms_total, double* ms_step, doubl
   if (ms_proj) *ms_proj = tproj;
   if (n_step) *n_step = static_cast<int>(S->ev_step.size());
   if (n_proj) *n_proj = static_cast<int>(S->ev_proj.size());
+  return ACHR_OK;
+}
+
+int achr_gemm_f32(int M, int N, int K, const float* A, long sa_m, long sa_k, const float* B, long sb_k, long sb_n,
+                  float* C, long sc_m, long sc_n, float alpha, float beta, int prec) {
+  if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return fail(ACHR_E_ARG, "bad GEMM arguments");
+  if ((sa_m != 1 && sa_k != 1) || (sb_k != 1 && sb_n != 1)) return fail(ACHR_E_ARG, "each operand needs a unit stride");
+  auto extent = [](long r, long c, long sr, long sc) { return (r - 1) * sr + (c - 1) * sc + 1; };
+  const size_t na = static_cast<size_t>(extent(M, K, sa_m, sa_k));
+  const size_t nb = static_cast<size_t>(extent(K, N, sb_k, sb_n));
+  const size_t nc = static_cast<size_t>(extent(M, N, sc_m, sc_n));
+  DevBuf<float> dA, dB, dC;
+  HIP_TRY(dA.upload(A, na, nullptr));
+  HIP_TRY(dB.upload(B, nb, nullptr));
+  HIP_TRY(dC.upload(C, nc, nullptr));
+  achr::GemmArgs g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = dA.p;
+  g.sa_m = sa_m;
+  g.sa_k = sa_k;
+  g.B = dB.p;
+  g.sb_k = sb_k;
+  g.sb_n = sb_n;
+  g.C = dC.p;
+  g.sc_m = sc_m;
+  g.sc_n = sc_n;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.epi = achr::kEpiStoreF32;
+  const hipError_t e = achr::launch_gemm(g, prec, M, nullptr);
+  if (e != hipSuccess) return fail(ACHR_E_ARG, "gemm launch: %s", hipGetErrorString(e));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(C, dC.p, sizeof(float) * nc, hipMemcpyDeviceToHost));
   return ACHR_OK;
 }
 

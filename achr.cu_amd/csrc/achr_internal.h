@@ -33,6 +33,7 @@ struct StepArgs {
   int4* st;              // {attempt, s, phase, consecutive misses}
   int4* stats;           // {misses, reprojections, checks, flags}
   int* flag;             // batched mode: 1 if chain b asks for a reprojection this round
+  float* R32;            // batched mode, optional: residual S x of each check, P x m
   double* out;           // n x P reaction major
   double* trace;
   int trace_chains, trace_thin, trace_len;
@@ -57,5 +58,7 @@ hipError_t launch_step(const StepArgs& a, const Schedule& s, hipStream_t stream)
 hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStream_t stream);
 hipError_t launch_project(const double* Ncm, int n, int d, double* X, double* T, const int* idx,
                           const int* count, int P, hipStream_t stream);
+// Elementwise FP64 -> FP32 copy of `count` values.
+hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_t stream);
 
 }  // namespace achr

@@ -11,6 +11,8 @@
 // the sequential dot product the host oracle computes. (On gfx950 FP64 vector
 // FMA has the same peak as FP64 MFMA, so nothing is lost by staying on the
 // VALU with a fixed accumulation order.)
+#include <algorithm>
+
 #include "achr_internal.h"
 
 namespace achr {
@@ -135,7 +137,20 @@ __global__ __launch_bounds__(1024) void compact_kernel(const int* __restrict__ f
   if (threadIdx.x == 0) *count = base;
 }
 
+__global__ void to_f32_kernel(const double* __restrict__ src, float* __restrict__ dst, size_t count) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < count;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    dst[i] = static_cast<float>(src[i]);
+}
+
 }  // namespace
+
+hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((count + 255) / 256, 65536);
+  hipLaunchKernelGGL(to_f32_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, src, dst, count);
+  return hipGetLastError();
+}
 
 hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStream_t stream) {
   hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, stream, flag, P, idx, count);

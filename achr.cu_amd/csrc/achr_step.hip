@@ -299,12 +299,14 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
         for (int k = 0; k < E; ++k) xrow.st(lane + k * TPC, x[k]);
         group_sync<TPC>();
         double rho = 0.0;
+        float* rrow = (a.mode == 2 && a.R32) ? a.R32 + static_cast<size_t>(b) * a.m : nullptr;
         for (int j = lane; j < a.m; j += TPC) {
           double acc = 0.0;
           const int p1 = a.rowptr[j + 1];
           for (int p = a.rowptr[j]; p < p1; ++p) acc = fma(a.vals[p], xrow.ld(a.colidx[p]), acc);
           const double av = fabs(acc);
           if (av > rho) rho = av;
+          if (rrow) rrow[j] = static_cast<float>(acc);
         }
         rho = group_max<TPC>(rho, red + 3 * kRedSlots);
         const bool reproj = rho > a.reprojTol && d > 0;

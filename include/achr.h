@@ -103,7 +103,26 @@ typedef struct achr_options {
   int reproject_mode;   /* 0 = auto, 1 = in-kernel per chain, 2 = batched GEMM rounds */
   int trace_chains;     /* chains whose thinned trajectory is recorded (0 = none) */
   int trace_thin;       /* record every trace_thin accepted steps */
+  /* Arithmetic of the batched projection rounds (reproject_mode 2):
+   *   ACHR_PROJ_FP64       x <- N (N^T x), FP64 FMA in the reference's loop
+   *                        order (bit-identical to the oracle);
+   *   ACHR_PROJ_RES_BF16X3 x <- x - S+ (S x): the FP64 residual S x of the
+   *                        check, contracted with the pseudo-inverse S+ on the
+   *                        matrix cores in split-BF16x3 with FP32 accumulation;
+   *   ACHR_PROJ_RES_F32    the same contraction with FP32 MFMA;
+   *   ACHR_PROJ_NULL_F32   x <- N (N^T x) with FP32 MFMA (x rounded to FP32);
+   *   ACHR_PROJ_NULL_BF16X3 the same in split-BF16x3.
+   * In exact arithmetic all five are the orthogonal projection onto null(S). */
+  int proj_prec;
+  int fp64_every;       /* with a reduced-precision proj_prec: every fp64_every-th
+                           round uses ACHR_PROJ_FP64 (10 = every 100 steps) */
 } achr_options;
+
+#define ACHR_PROJ_FP64 0
+#define ACHR_PROJ_RES_BF16X3 1
+#define ACHR_PROJ_RES_F32 2
+#define ACHR_PROJ_NULL_F32 3
+#define ACHR_PROJ_NULL_BF16X3 4
 
 /* Fills the reference defaults. */
 void achr_options_default(achr_options *opts);
@@ -181,6 +200,13 @@ void achr_free(void *p);
 /* Philox4x32-10 block of the library's stream (host implementation of the
  * same header the kernels use). */
 void achr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* The library's matrix-core contraction on host arrays (copied to and from
+ * the device): C = alpha * A B + beta * C with A(i, k) = A[i*sa_m + k*sa_k],
+ * B(k, j) = B[k*sb_k + j*sb_n], C(i, j) = C[i*sc_m + j*sc_n] (one stride of
+ * each operand must be 1). prec: 0 FP32 MFMA, 1 split-BF16x3, 2 BF16. */
+int achr_gemm_f32(int M, int N, int K, const float *A, long sa_m, long sa_k, const float *B, long sb_k, long sb_n,
+                  float *C, long sc_m, long sc_n, float alpha, float beta, int prec);
 
 #ifdef __cplusplus
 }
