@@ -61,6 +61,20 @@ SIGNATURES = {
     "achr_read_warmup_csv": (_I, [C.c_char_p, _I, C.POINTER(_PD), _PI]),
     "achr_free": (None, [_P]),
     "achr_philox4x32_10": (None, [_PU32, _PU32, _PU32]),
+    "achr_dirnet_create": (_I, [_I, _I, _I, C.c_uint64, C.POINTER(_P)]),
+    "achr_dirnet_destroy": (None, [_P]),
+    "achr_dirnet_param_count": (_I, [_P, C.POINTER(C.c_long)]),
+    "achr_dirnet_get_params": (_I, [_P, C.POINTER(C.c_float)]),
+    "achr_dirnet_set_params": (_I, [_P, C.POINTER(C.c_float)]),
+    "achr_dirnet_set_target": (_I, [_P, C.POINTER(C.c_float)]),
+    "achr_dirnet_loss_grad": (_I, [_P, C.POINTER(C.c_float), _I, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "achr_dirnet_train": (_I, [_P, _I, _I, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
+    "achr_dirnet_sample": (_I, [_P, C.POINTER(C.c_float), _I, _I, C.POINTER(C.c_float)]),
+    "achr_learn_options_default": (None, [_P]),
+    "achr_sampler_learn_directions": (_I, [_P, _P, _PD]),
+    "achr_sampler_set_directions": (_I, [_P, _PD, _I]),
+    "achr_sampler_use_directions": (_I, [_P, _I]),
+    "achr_sampler_set_run_options": (_I, [_P, _I, _I, _I]),
     "achr_gemm_f32": (_I, [_I, _I, _I, C.POINTER(C.c_float), C.c_long, C.c_long, C.POINTER(C.c_float), C.c_long,
                            C.c_long, C.POINTER(C.c_float), C.c_long, C.c_long, C.c_float, C.c_float, _I]),
 }
@@ -82,6 +96,21 @@ class AchrOptions(C.Structure):
         ("trace_thin", C.c_int),
         ("proj_prec", C.c_int),
         ("fp64_every", C.c_int),
+    ]
+
+
+class AchrLearnOptions(C.Structure):
+    """Mirror of ``achr_learn_options`` (include/achr.h)."""
+
+    _fields_ = [
+        ("rank", C.c_int),
+        ("width", C.c_int),
+        ("iters", C.c_int),
+        ("batch", C.c_int),
+        ("lr", C.c_float),
+        ("bank_size", C.c_int),
+        ("isotropic", C.c_int),
+        ("seed", C.c_uint64),
     ]
 
 

@@ -12,7 +12,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import AchrOptions, check, lib
+from ._lib import AchrLearnOptions, AchrOptions, check, lib
 
 
 def _dp(a: np.ndarray):
@@ -205,7 +205,7 @@ class Sampler:
 
     def trace(self) -> np.ndarray:
         tc = min(self.opts.trace_chains, self.P)
-        L = self.S // self.opts.trace_thin
+        L = self.S // self.opts.trace_thin if self.opts.trace_thin > 0 else 0
         tr = np.empty((tc, L, self.n))
         check(lib().achr_sample_trace(self._h, _dp(tr)), "achr_sample_trace")
         return tr
@@ -221,6 +221,28 @@ class Sampler:
     def sample(self, P: int, S: int, seed: int = 0, file: int = 0, chain0: int = 0) -> np.ndarray:
         self.run(P, S, seed, file, chain0)
         return self.points()
+
+    def learn_directions(self, rank: int = 512, width: int = 512, iters: int = 400, batch: int = 2048,
+                         lr: float = 1e-3, bank_size: int = 0, isotropic: bool = False, seed: int = 7) -> dict:
+        """Learned direction bank from the last traced run; switches the sampler to it."""
+        o = AchrLearnOptions(rank, width, iters, batch, lr, bank_size, 1 if isotropic else 0, seed)
+        rep = np.zeros(8)
+        check(lib().achr_sampler_learn_directions(self._h, C.byref(o), _dp(rep)), "achr_sampler_learn_directions")
+        keys = ["free_dim", "samples", "loss_start", "loss_end", "sigma_perp2", "captured", "ms_total", "ms_train"]
+        return dict(zip(keys, rep.tolist()))
+
+    def set_directions(self, bank: np.ndarray) -> None:
+        b = np.asfortranarray(bank, dtype=np.float64)
+        flat = np.ascontiguousarray(b.ravel(order="F"))
+        check(lib().achr_sampler_set_directions(self._h, _dp(flat), b.shape[1]), "achr_sampler_set_directions")
+
+    def set_run_options(self, trace_chains: int, trace_thin: int, check_every: int) -> None:
+        check(lib().achr_sampler_set_run_options(self._h, trace_chains, trace_thin, check_every),
+              "achr_sampler_set_run_options")
+        self.opts.trace_chains, self.opts.trace_thin, self.opts.check_every = trace_chains, trace_thin, check_every
+
+    def use_directions(self, on: bool) -> None:
+        check(lib().achr_sampler_use_directions(self._h, 1 if on else 0), "achr_sampler_use_directions")
 
 
 def philox4x32_10(ctr, key) -> tuple:
@@ -276,3 +298,59 @@ def gemm_f32(A, B, prec: int = 0, alpha: float = 1.0, C=None, beta: float = 0.0,
     check(lib().achr_gemm_f32(M, N, K, A.ctypes.data_as(P), sa_m, sa_k, B.ctypes.data_as(P), sb_k, sb_n,
                               out.ctypes.data_as(P), N, 1, alpha, beta, prec), "achr_gemm_f32")
     return out
+
+
+def _fp(a):
+    return a.ctypes.data_as(_ctypes.POINTER(_ctypes.c_float))
+
+
+class DirNet:
+    """The direction network (include/achr.h achr_dirnet_*)."""
+
+    def __init__(self, latent: int, width: int, outdim: int, seed: int = 0):
+        h = C.c_void_p()
+        check(lib().achr_dirnet_create(latent, width, outdim, C.c_uint64(seed), C.byref(h)), "achr_dirnet_create")
+        self._h = h
+        self.L, self.H, self.O = latent, width, outdim
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().achr_dirnet_destroy(h)
+            self._h = None
+
+    def param_count(self) -> int:
+        c = C.c_long()
+        check(lib().achr_dirnet_param_count(self._h, C.byref(c)), "param_count")
+        return c.value
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.param_count(), dtype=np.float32)
+        check(lib().achr_dirnet_get_params(self._h, _fp(p)), "get_params")
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().achr_dirnet_set_params(self._h, _fp(p)), "set_params")
+
+    def set_target(self, Ct: np.ndarray) -> None:
+        Ct = np.ascontiguousarray(Ct, dtype=np.float32)
+        check(lib().achr_dirnet_set_target(self._h, _fp(Ct)), "set_target")
+
+    def loss_grad(self, Z: np.ndarray):
+        Z = np.ascontiguousarray(Z, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.empty(self.param_count(), dtype=np.float32)
+        check(lib().achr_dirnet_loss_grad(self._h, _fp(Z), Z.shape[0], _fp(loss), _fp(g)), "loss_grad")
+        return float(loss[0]), g
+
+    def train(self, iters: int, batch: int, lr: float, seed: int = 0) -> np.ndarray:
+        losses = np.zeros(max(iters, 1), dtype=np.float32)
+        check(lib().achr_dirnet_train(self._h, iters, batch, lr, C.c_uint64(seed), _fp(losses)), "train")
+        return losses[:iters]
+
+    def sample(self, Z: np.ndarray, prec: int = 0) -> np.ndarray:
+        Z = np.ascontiguousarray(Z, dtype=np.float32)
+        V = np.empty((Z.shape[0], self.O), dtype=np.float32)
+        check(lib().achr_dirnet_sample(self._h, _fp(Z), Z.shape[0], prec, _fp(V)), "sample")
+        return V

@@ -108,7 +108,13 @@ def make_model(m: int, n: int, seed: int, *, irrev_frac: float, n_fixed: int, wi
     irrev = rng.random(n) < irrev_frac
     lb = np.where(irrev & (x0 > 0), np.maximum(lb, 0.0), lb)
     if len(fixed):
+        # 4-decimal fixed values when x0 reaches them exactly (they then fit
+        # a fixed-format MPS field); otherwise x0's own values, since fixed
+        # reactions with dependent null-space rows cannot take arbitrary
+        # rounded values and stay on S x = 0.
         v = np.round(x0[fixed], 4)
+        if np.max(np.abs(x0[fixed] - v)) > 1e-12:
+            v = x0[fixed]
         lb[fixed] = v
         ub[fixed] = v
     if S is None and n <= 4000:

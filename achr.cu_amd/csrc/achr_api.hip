@@ -19,10 +19,14 @@
 
 #include "../../include/achr.h"
 #include "achr_gemm.h"
+#include "achr_host.h"
 #include "achr_internal.h"
 #include "achr_philox.h"
 
-namespace {
+using achr_host::DevBuf;
+using achr_host::fail;
+
+namespace achr_host {
 
 thread_local std::string g_err;
 
@@ -36,82 +40,11 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                          \
-  do {                                                                                         \
-    hipError_t e_ = (expr);                                                                    \
-    if (e_ != hipSuccess) return fail(ACHR_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t count = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    count = 0;
-  }
-  hipError_t ensure(size_t n) {
-    if (n <= count && p) return hipSuccess;
-    release();
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
-    if (e == hipSuccess) count = n;
-    return e;
-  }
-  hipError_t upload(const T* h, size_t n, hipStream_t s) {
-    hipError_t e = ensure(n);
-    if (e != hipSuccess || n == 0) return e;
-    e = hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
-  }
-};
-
-}  // namespace
-
-struct achr_model {
-  int n = 0, m = 0;
-  std::vector<int> rowptr, colidx;
-  std::vector<double> vals, lb, ub;
-  std::vector<double> N;  // n x d column major
-  int d = -1;             // -1: not computed yet
-};
-
-struct achr_sampler {
-  int n = 0, m = 0, W = 0, d = 0;
-  achr_options opt{};
-  achr::Schedule sched{};
-  int mode = 1;
-  hipStream_t stream = nullptr;
-  std::vector<double> c0;
-  DevBuf<double> warm, lb, ub, c0d, vals, Ncm, Nrm;
-  DevBuf<int> rowptr, colidx;
-  // reduced-precision projection operands (FP32 copies)
-  DevBuf<float> Sp32;   // pseudo-inverse S+, n x m column major (i contiguous)
-  DevBuf<float> N32;    // N, n x d column major
-  DevBuf<float> R32, T32;  // per-chain residuals (P x m) and N^T x (P x d)
-  int rounds_fp64 = 0, rounds_low = 0;
-  // run state
-  int P = 0, S = 0;
-  DevBuf<double> X, C, T, out, trace;
-  DevBuf<int4> st, stats;
-  DevBuf<int> flag, idx, count;
-  int trace_len = 0;
-  // timing
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, int>> ev_step, ev_proj;  // indices into ev_pool
-  bool have_run = false;
-};
+}  // namespace achr_host
 
 extern "C" {
 
-const char* achr_last_error(void) { return g_err.c_str(); }
+const char* achr_last_error(void) { return achr_host::g_err.c_str(); }
 const char* achr_version(void) { return "achr-mi355x 0.1.0 (gfx950)"; }
 
 void achr_free(void* p) { std::free(p); }
@@ -604,6 +537,7 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
     return fail(ACHR_E_ARG, "reduced-precision projection runs in batched rounds (reproject_mode 2)");
   auto* S = new (std::nothrow) achr_sampler();
   if (!S) return fail(ACHR_E_NOMEM, "sampler allocation");
+  S->model = M;
   S->n = M->n;
   S->m = M->m;
   S->W = W;
@@ -810,9 +744,10 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   achr::StepArgs a{};
   a.n = n;
   a.m = S->m;
-  a.W = S->W;
+  a.W = S->use_dirs ? S->K : S->W;
   a.d = S->d;
-  a.warm = S->warm.p;
+  a.warm = S->use_dirs ? S->dirs.p : S->warm.p;
+  a.dir_mode = S->use_dirs;
   a.lb = S->lb.p;
   a.ub = S->ub.p;
   a.c0 = S->c0d.p;
@@ -844,6 +779,7 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   a.out = S->out.p;
   a.trace = S->trace_len > 0 ? S->trace.p : nullptr;
   a.trace_chains = tr_chains;
+  S->trace_chains_run = S->trace_len > 0 ? tr_chains : 0;
   a.trace_thin = S->opt.trace_thin;
   a.trace_len = S->trace_len;
 
@@ -978,7 +914,7 @@ int achr_sample_trace(achr_sampler* S, double* trace) {
   if (!S->have_run) return fail(ACHR_E_STATE, "no run");
   if (S->trace_len == 0) return fail(ACHR_E_STATE, "no trace recorded (trace_chains / trace_thin)");
   HIP_TRY(hipStreamSynchronize(S->stream));
-  const size_t cnt = static_cast<size_t>(std::min(S->opt.trace_chains, S->P)) * S->trace_len * S->n;
+  const size_t cnt = static_cast<size_t>(S->trace_chains_run) * S->trace_len * S->n;
   HIP_TRY(hipMemcpy(trace, S->trace.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
   return ACHR_OK;
 }

@@ -1,0 +1,95 @@
+// Host-side handle types and error helpers shared by the C ABI sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/achr.h"
+#include "achr_internal.h"
+
+namespace achr_host {
+
+int fail(int code, const char* fmt, ...);
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= count && p) return hipSuccess;
+    release();
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+    if (e == hipSuccess) count = n;
+    return e;
+  }
+  hipError_t upload(const T* h, size_t n, hipStream_t s) {
+    hipError_t e = ensure(n);
+    if (e != hipSuccess || n == 0) return e;
+    e = hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+};
+
+}  // namespace achr_host
+
+#define HIP_TRY(expr)                                                                                      \
+  do {                                                                                                     \
+    hipError_t e_ = (expr);                                                                                \
+    if (e_ != hipSuccess) return achr_host::fail(ACHR_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct achr_model {
+  int n = 0, m = 0;
+  std::vector<int> rowptr, colidx;
+  std::vector<double> vals, lb, ub;
+  std::vector<double> N;  // n x d column major
+  int d = -1;             // -1: not computed yet
+};
+
+struct achr_sampler {
+  const achr_model* model = nullptr;
+  int n = 0, m = 0, W = 0, d = 0;
+  achr_options opt{};
+  achr::Schedule sched{};
+  int mode = 1;
+  hipStream_t stream = nullptr;
+  std::vector<double> c0;
+  achr_host::DevBuf<double> warm, lb, ub, c0d, vals, Ncm, Nrm;
+  achr_host::DevBuf<int> rowptr, colidx;
+  // reduced-precision projection operands (FP32 copies)
+  achr_host::DevBuf<float> Sp32;      // pseudo-inverse S+, n x m column major (i contiguous)
+  achr_host::DevBuf<float> N32;       // N, n x d column major
+  achr_host::DevBuf<float> R32, T32;  // per-chain residuals (P x m) and N^T x (P x d)
+  int rounds_fp64 = 0, rounds_low = 0;
+  // direction bank (learned sampler)
+  achr_host::DevBuf<double> dirs;     // n x K column major, unit columns
+  int K = 0;
+  int use_dirs = 0;
+  // run state
+  int P = 0, S = 0;
+  achr_host::DevBuf<double> X, C, T, out, trace;
+  achr_host::DevBuf<int4> st, stats;
+  achr_host::DevBuf<int> flag, idx, count;
+  int trace_len = 0;
+  int trace_chains_run = 0;
+  // timing
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_step, ev_proj;  // indices into ev_pool
+  bool have_run = false;
+};

@@ -35,6 +35,7 @@ struct StepArgs {
   int* flag;             // batched mode: 1 if chain b asks for a reprojection this round
   float* R32;            // batched mode, optional: residual S x of each check, P x m
   double* out;           // n x P reaction major
+  int dir_mode;          // 1: warm holds a bank of unit directions (no centre)
   double* trace;
   int trace_chains, trace_thin, trace_len;
 };
@@ -58,6 +59,24 @@ hipError_t launch_step(const StepArgs& a, const Schedule& s, hipStream_t stream)
 hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStream_t stream);
 hipError_t launch_project(const double* Ncm, int n, int d, double* X, double* T, const int* idx,
                           const int* count, int P, hipStream_t stream);
+// C(:, q) = A B(:, q) in FP64, k ascending from +0.0 (bitwise sequential):
+// A(r, k) = A[r*sar + k*sak] (M x K), column q of B at B + q*ldb, column q of C at C + q*ldc.
+hipError_t launch_dgemm_cols(int M, int K, const double* A, long sar, long sak, const double* B, long ldb,
+                             double* C, long ldc, const int* count, int max_cols, hipStream_t stream);
+// Normalises each of the K columns (length n) of A to unit Euclidean norm.
+hipError_t launch_normalize_cols(double* A, int n, int K, hipStream_t stream);
+// Column means of the rows x cols row-major FP64 matrix X (mean[j]).
+hipError_t launch_col_means(const double* X, long rows, int cols, double* mean, hipStream_t stream);
+// Y[r, j] -= v[j] for the rows x cols row-major FP32 matrix Y.
+hipError_t launch_sub_row(float* Y, long rows, int cols, const float* v, hipStream_t stream);
+// Standard normal FP32 values from the Philox stream.
+hipError_t launch_gauss(float* z, size_t count, uint64_t seed, uint32_t stream_id, hipStream_t stream);
+// y = alpha * y + beta * x (FP32), x and y of `count` values.
+hipError_t launch_axpby(float* y, const float* x, float alpha, float beta, size_t count, hipStream_t stream);
+// FP32 -> FP64 copy.
+hipError_t launch_to_f64(const float* src, double* dst, size_t count, hipStream_t stream);
+// Sum of squares of `count` FP32 values, accumulated in FP64 (synchronous).
+hipError_t sum_squares(const float* x, size_t count, double* out);
 // Elementwise FP64 -> FP32 copy of `count` values.
 hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_t stream);
 

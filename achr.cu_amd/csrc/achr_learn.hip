@@ -1,0 +1,552 @@
+// Learned direction law: network handles and the sample -> rounding map ->
+// direction bank pipeline (declared in include/achr.h). Host code; the
+// device work is the matrix-core GEMM, the FP64 column GEMM, the network's
+// kernels and rocSOLVER for one small symmetric eigenproblem.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "achr_dirnet.h"
+#include "achr_gemm.h"
+#include "achr_host.h"
+#include "achr_internal.h"
+
+using achr_host::DevBuf;
+using achr_host::fail;
+
+struct achr_dirnet {
+  achr::DirNet net;
+  bool has_target = false;
+  ~achr_dirnet() { net.release(); }
+};
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int upload_target(achr_dirnet* d, const float* C) {
+  achr::DirNet& n = d->net;
+  HIP_TRY(n.ensure_cov());
+  HIP_TRY(hipMemcpy(n.target, C, sizeof(float) * n.O * n.O, hipMemcpyHostToDevice));
+  double s = 0.0;
+  for (size_t i = 0; i < static_cast<size_t>(n.O) * n.O; ++i) s += static_cast<double>(C[i]) * C[i];
+  n.target_norm2 = static_cast<float>(s > 0.0 ? s : 1.0);
+  d->has_target = true;
+  return ACHR_OK;
+}
+
+// loss from the error matrix left in net.cov by backward()
+int read_loss(achr::DirNet& n, float* loss) {
+  std::vector<float> E(static_cast<size_t>(n.O) * n.O);
+  HIP_TRY(hipMemcpy(E.data(), n.cov, sizeof(float) * E.size(), hipMemcpyDeviceToHost));
+  double s = 0.0;
+  for (float e : E) s += static_cast<double>(e) * e;
+  *loss = static_cast<float>(s / n.target_norm2);
+  return ACHR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int achr_dirnet_create(int latent, int width, int outdim, uint64_t seed, achr_dirnet** out) {
+  if (!out || latent <= 0 || width <= 0 || outdim <= 0) return fail(ACHR_E_ARG, "bad network shape");
+  *out = nullptr;
+  auto* d = new (std::nothrow) achr_dirnet();
+  if (!d) return fail(ACHR_E_NOMEM, "network allocation");
+  hipError_t e = d->net.alloc(latent, width, outdim);
+  if (e != hipSuccess) {
+    delete d;
+    return fail(ACHR_E_DEVICE, "network allocation: %s", hipGetErrorString(e));
+  }
+  std::vector<float> host;
+  d->net.init_params(seed, host);
+  e = hipMemcpy(d->net.params, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    delete d;
+    return fail(ACHR_E_DEVICE, "network upload: %s", hipGetErrorString(e));
+  }
+  *out = d;
+  return ACHR_OK;
+}
+
+void achr_dirnet_destroy(achr_dirnet* net) { delete net; }
+
+int achr_dirnet_param_count(const achr_dirnet* d, long* count) {
+  if (!d || !count) return fail(ACHR_E_ARG, "NULL argument");
+  *count = static_cast<long>(d->net.param_count());
+  return ACHR_OK;
+}
+
+int achr_dirnet_get_params(const achr_dirnet* d, float* params) {
+  if (!d || !params) return fail(ACHR_E_ARG, "NULL argument");
+  HIP_TRY(hipMemcpy(params, d->net.params, sizeof(float) * d->net.param_count(), hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
+int achr_dirnet_set_params(achr_dirnet* d, const float* params) {
+  if (!d || !params) return fail(ACHR_E_ARG, "NULL argument");
+  HIP_TRY(hipMemcpy(d->net.params, params, sizeof(float) * d->net.param_count(), hipMemcpyHostToDevice));
+  return ACHR_OK;
+}
+
+int achr_dirnet_set_target(achr_dirnet* d, const float* C) {
+  if (!d || !C) return fail(ACHR_E_ARG, "NULL argument");
+  return upload_target(d, C);
+}
+
+int achr_dirnet_loss_grad(achr_dirnet* d, const float* Z, int B, float* loss, float* grads) {
+  if (!d || !Z || B <= 0) return fail(ACHR_E_ARG, "bad arguments");
+  if (!d->has_target) return fail(ACHR_E_STATE, "no target covariance");
+  achr::DirNet& n = d->net;
+  HIP_TRY(n.ensure_batch(B));
+  HIP_TRY(hipMemcpy(n.z, Z, sizeof(float) * B * n.L, hipMemcpyHostToDevice));
+  HIP_TRY(n.forward(B, achr::kPrecF32, nullptr));
+  HIP_TRY(n.backward(B, n.target_norm2, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  if (loss) {
+    const int rc = read_loss(n, loss);
+    if (rc != ACHR_OK) return rc;
+  }
+  if (grads) HIP_TRY(hipMemcpy(grads, n.grads, sizeof(float) * n.param_count(), hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
+int achr_dirnet_train(achr_dirnet* d, int iters, int batch, float lr, uint64_t seed, float* losses) {
+  if (!d || iters < 0 || batch <= 0 || !(lr > 0.0f)) return fail(ACHR_E_ARG, "bad training arguments");
+  if (!d->has_target) return fail(ACHR_E_STATE, "no target covariance");
+  achr::DirNet& n = d->net;
+  HIP_TRY(n.ensure_batch(batch));
+  for (int it = 0; it < iters; ++it) {
+    HIP_TRY(n.gaussian_latents(batch, seed, static_cast<uint32_t>(it), nullptr));
+    HIP_TRY(n.forward(batch, achr::kPrecF32, nullptr));
+    HIP_TRY(n.backward(batch, n.target_norm2, nullptr));
+    if (losses) {
+      HIP_TRY(hipDeviceSynchronize());
+      const int rc = read_loss(n, losses + it);
+      if (rc != ACHR_OK) return rc;
+    }
+    HIP_TRY(n.adam_step(lr, ++n.adam_t, nullptr));
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  return ACHR_OK;
+}
+
+int achr_dirnet_sample(achr_dirnet* d, const float* Z, int B, int prec, float* V) {
+  if (!d || !Z || !V || B <= 0 || (prec != achr::kPrecF32 && prec != achr::kPrecBF16))
+    return fail(ACHR_E_ARG, "bad arguments");
+  achr::DirNet& n = d->net;
+  HIP_TRY(n.ensure_batch(B));
+  HIP_TRY(hipMemcpy(n.z, Z, sizeof(float) * B * n.L, hipMemcpyHostToDevice));
+  HIP_TRY(n.forward(B, prec, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(V, n.v, sizeof(float) * B * n.O, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
+void achr_learn_options_default(achr_learn_options* o) {
+  if (!o) return;
+  o->rank = 512;
+  o->width = 512;
+  o->iters = 400;
+  o->batch = 2048;
+  o->lr = 1e-3f;
+  o->bank_size = 0;  // 0: 2n directions, like the reference's 2n warmup points
+  o->isotropic = 0;
+  o->seed = 7;
+}
+
+int achr_sampler_set_directions(achr_sampler* S, const double* bank, int K) {
+  if (!S || !bank || K <= 0) return fail(ACHR_E_ARG, "bad bank");
+  if (S->sched.tpc < 64 && 8.0 * S->n * K >= 2147483648.0) return fail(ACHR_E_ARG, "bank too large for the schedule");
+  HIP_TRY(S->dirs.upload(bank, static_cast<size_t>(S->n) * K, nullptr));
+  S->K = K;
+  return ACHR_OK;
+}
+
+int achr_sampler_set_run_options(achr_sampler* S, int trace_chains, int trace_thin, int check_every) {
+  if (!S || trace_chains < 0 || (trace_chains > 0 && trace_thin <= 0) || check_every <= 0)
+    return fail(ACHR_E_ARG, "bad run options");
+  S->opt.trace_chains = trace_chains;
+  S->opt.trace_thin = trace_thin;
+  S->opt.check_every = check_every;
+  return ACHR_OK;
+}
+
+int achr_sampler_use_directions(achr_sampler* S, int on) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  if (on && S->K == 0) return fail(ACHR_E_STATE, "no direction bank");
+  S->use_dirs = on ? 1 : 0;
+  return ACHR_OK;
+}
+
+// Orthonormal basis N_f of {v in span(N): v_i = 0 for fixed reactions},
+// n x df column major FP64 on the device.
+static int free_basis(const achr_model* M, const double* dN, DevBuf<double>& Nf, int* df_out) {
+  const int n = M->n, d = M->d;
+  std::vector<int> fixed;
+  for (int i = 0; i < n; ++i)
+    if (M->lb[i] == M->ub[i]) fixed.push_back(i);
+  const int nf = static_cast<int>(fixed.size());
+  if (nf == 0) {
+    HIP_TRY(Nf.ensure(static_cast<size_t>(n) * d));
+    HIP_TRY(hipMemcpy(Nf.p, dN, sizeof(double) * n * d, hipMemcpyDeviceToDevice));
+    *df_out = d;
+    return ACHR_OK;
+  }
+  // G = N_F^T N_F (d x d) on the host from the nf rows, eigenvectors on the GPU.
+  std::vector<double> NF(static_cast<size_t>(nf) * d);
+  for (int r = 0; r < nf; ++r)
+    for (int j = 0; j < d; ++j) NF[static_cast<size_t>(r) + static_cast<size_t>(j) * nf] = M->N[static_cast<size_t>(fixed[r]) + static_cast<size_t>(j) * n];
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return fail(ACHR_E_SOLVER, "rocblas_create_handle");
+  DevBuf<double> dNF, dG, dD, dE, dZ;
+  DevBuf<int> dinfo;
+  int rc = ACHR_OK;
+  std::vector<double> ev(d);
+  std::vector<double> Gh(static_cast<size_t>(d) * d);
+  const double one = 1.0, zero = 0.0;
+  do {
+    if (dNF.upload(NF.data(), NF.size(), nullptr) != hipSuccess || dG.ensure(static_cast<size_t>(d) * d) != hipSuccess ||
+        dD.ensure(d) != hipSuccess || dE.ensure(d) != hipSuccess || dinfo.ensure(1) != hipSuccess) {
+      rc = fail(ACHR_E_NOMEM, "free basis buffers");
+      break;
+    }
+    if (rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, d, d, nf, &one, dNF.p, nf, dNF.p, nf,
+                      &zero, dG.p, d) != rocblas_status_success ||
+        rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, d, dG.p, d, dD.p, dE.p, dinfo.p) !=
+            rocblas_status_success) {
+      rc = fail(ACHR_E_SOLVER, "free basis eigenproblem");
+      break;
+    }
+    if (hipMemcpy(ev.data(), dD.p, sizeof(double) * d, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(Gh.data(), dG.p, sizeof(double) * Gh.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(ACHR_E_DEVICE, "free basis download");
+      break;
+    }
+    double lmax = 0.0;
+    for (double v : ev) lmax = std::max(lmax, std::fabs(v));
+    const double tol = std::max(1.0, static_cast<double>(d)) * lmax * 2.2204e-16;
+    std::vector<int> keep;
+    for (int i = 0; i < d; ++i)
+      if (std::fabs(ev[i]) < tol) keep.push_back(i);
+    const int df = static_cast<int>(keep.size());
+    if (df == 0) {
+      rc = fail(ACHR_E_STATE, "the fixed reactions leave no free direction");
+      break;
+    }
+    std::vector<double> Zh(static_cast<size_t>(d) * df);
+    for (int j = 0; j < df; ++j)
+      std::copy(Gh.begin() + static_cast<size_t>(keep[j]) * d, Gh.begin() + static_cast<size_t>(keep[j] + 1) * d,
+                Zh.begin() + static_cast<size_t>(j) * d);
+    if (dZ.upload(Zh.data(), Zh.size(), nullptr) != hipSuccess || Nf.ensure(static_cast<size_t>(n) * df) != hipSuccess) {
+      rc = fail(ACHR_E_NOMEM, "free basis");
+      break;
+    }
+    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, df, d, &one, dN, n, dZ.p, d, &zero, Nf.p,
+                      n) != rocblas_status_success || hipDeviceSynchronize() != hipSuccess) {
+      rc = fail(ACHR_E_SOLVER, "free basis product");
+      break;
+    }
+    *df_out = df;
+  } while (false);
+  rocblas_destroy_handle(h);
+  return rc;
+}
+
+// Cholesky-QR (twice) of the df x q FP64 matrix G (column major, host): G <- Q.
+static bool chol_qr(std::vector<double>& G, int rows, int q) {
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<double> A(static_cast<size_t>(q) * q, 0.0);
+    for (int a = 0; a < q; ++a)
+      for (int b = a; b < q; ++b) {
+        const double* ga = G.data() + static_cast<size_t>(a) * rows;
+        const double* gb = G.data() + static_cast<size_t>(b) * rows;
+        double s = 0.0;
+        for (int i = 0; i < rows; ++i) s += ga[i] * gb[i];
+        A[a + static_cast<size_t>(b) * q] = s;
+        A[b + static_cast<size_t>(a) * q] = s;
+      }
+    // A = R^T R (upper R)
+    std::vector<double> R(static_cast<size_t>(q) * q, 0.0);
+    for (int j = 0; j < q; ++j) {
+      double s = A[j + static_cast<size_t>(j) * q];
+      for (int k = 0; k < j; ++k) s -= R[k + static_cast<size_t>(j) * q] * R[k + static_cast<size_t>(j) * q];
+      if (!(s > 0.0)) return false;
+      const double rjj = std::sqrt(s);
+      R[j + static_cast<size_t>(j) * q] = rjj;
+      for (int i = j + 1; i < q; ++i) {
+        double t = A[j + static_cast<size_t>(i) * q];
+        for (int k = 0; k < j; ++k) t -= R[k + static_cast<size_t>(j) * q] * R[k + static_cast<size_t>(i) * q];
+        R[j + static_cast<size_t>(i) * q] = t / rjj;
+      }
+    }
+    // G <- G R^{-1}, column by column
+    for (int j = 0; j < q; ++j) {
+      double* gj = G.data() + static_cast<size_t>(j) * rows;
+      for (int k = 0; k < j; ++k) {
+        const double r = R[k + static_cast<size_t>(j) * q];
+        const double* gk = G.data() + static_cast<size_t>(k) * rows;
+        for (int i = 0; i < rows; ++i) gj[i] -= r * gk[i];
+      }
+      const double inv = 1.0 / R[j + static_cast<size_t>(j) * q];
+      for (int i = 0; i < rows; ++i) gj[i] *= inv;
+    }
+  }
+  return true;
+}
+
+static achr::GemmArgs gemm(int M, int N, int K, const void* A, long sa_m, long sa_k, const float* B, long sb_k,
+                           long sb_n, void* C, long sc_m, long sc_n, float alpha = 1.0f, float beta = 0.0f,
+                           int epi = achr::kEpiStoreF32) {
+  achr::GemmArgs g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.sa_m = sa_m;
+  g.sa_k = sa_k;
+  g.B = B;
+  g.sb_k = sb_k;
+  g.sb_n = sb_n;
+  g.C = C;
+  g.sc_m = sc_m;
+  g.sc_n = sc_n;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.epi = epi;
+  return g;
+}
+
+#define GEMM_TRY(g, prec, rows)                                                               \
+  do {                                                                                        \
+    hipError_t e_ = achr::launch_gemm((g), (prec), (rows), nullptr);                          \
+    if (e_ != hipSuccess) return fail(ACHR_E_DEVICE, "gemm: %s", hipGetErrorString(e_));      \
+  } while (0)
+
+int achr_sampler_learn_directions(achr_sampler* S, const achr_learn_options* opts, double* report) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  achr_learn_options o;
+  achr_learn_options_default(&o);
+  if (opts) o = *opts;
+  const achr_model* M = S->model;
+  const int n = S->n;
+  if (S->d <= 0) return fail(ACHR_E_STATE, "empty null space");
+  const double t0 = now_ms();
+  DevBuf<double> Nf;
+  int df = 0;
+  int rc = free_basis(M, S->Ncm.p, Nf, &df);
+  if (rc != ACHR_OK) return rc;
+  const int K = o.bank_size > 0 ? o.bank_size : 2 * n;
+  if (S->sched.tpc < 64 && 8.0 * n * K >= 2147483648.0) return fail(ACHR_E_ARG, "bank too large for the schedule");
+  const int r = std::min(o.rank, df);
+  long Msamp = 0;
+  double loss0 = 0.0, loss1 = 0.0, sig2 = 1.0, captured = 0.0, train_ms = 0.0;
+  DevBuf<float> Q32;     // df x r column major (Q[i + j*df])
+  DevBuf<float> Wmix;    // K x df row major
+  HIP_TRY(Wmix.ensure(static_cast<size_t>(K) * df));
+  const uint64_t seed = o.seed;
+  if (o.isotropic || r <= 0) {
+    HIP_TRY(achr::launch_gauss(Wmix.p, static_cast<size_t>(K) * df, seed, 0xB0u, nullptr));
+  } else {
+    if (!S->have_run || S->trace_chains_run <= 0 || S->trace_len <= 0)
+      return fail(ACHR_E_STATE, "learning needs a traced run (trace_chains, trace_thin)");
+    Msamp = static_cast<long>(S->trace_chains_run) * S->trace_len;
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    // Samples in free null-space coordinates: Y = (X - mean) N_f, M x df FP32.
+    DevBuf<float> Nf32, Y, mf;
+    DevBuf<double> mean;
+    HIP_TRY(Nf32.ensure(static_cast<size_t>(n) * df));
+    HIP_TRY(achr::launch_to_f32(Nf.p, Nf32.p, static_cast<size_t>(n) * df, nullptr));
+    HIP_TRY(Y.ensure(static_cast<size_t>(Msamp) * df));
+    HIP_TRY(mean.ensure(n));
+    HIP_TRY(mf.ensure(df));
+    HIP_TRY(achr::launch_col_means(S->trace.p, Msamp, n, mean.p, nullptr));
+    {
+      achr::GemmArgs g = gemm(static_cast<int>(Msamp), df, n, S->trace.p, n, 1, Nf32.p, 1, n, Y.p, df, 1);
+      g.a_f64 = 1;
+      GEMM_TRY(g, achr::kPrecBF16x3, static_cast<int>(Msamp));
+      achr::GemmArgs gm = gemm(1, df, n, mean.p, n, 1, Nf32.p, 1, n, mf.p, df, 1);
+      gm.a_f64 = 1;
+      GEMM_TRY(gm, achr::kPrecBF16x3, 1);
+      HIP_TRY(achr::launch_sub_row(Y.p, Msamp, df, mf.p, nullptr));
+    }
+    // Randomised range finder: G = Y^T (Y Omega), Omega df x q Gaussian.
+    const int q = std::min(df, r + 16);
+    DevBuf<float> Om, P1, Gd;
+    HIP_TRY(Om.ensure(static_cast<size_t>(df) * q));
+    HIP_TRY(P1.ensure(static_cast<size_t>(Msamp) * q));
+    HIP_TRY(Gd.ensure(static_cast<size_t>(df) * q));
+    HIP_TRY(achr::launch_gauss(Om.p, static_cast<size_t>(df) * q, seed, 0xB1u, nullptr));
+    GEMM_TRY(gemm(static_cast<int>(Msamp), q, df, Y.p, df, 1, Om.p, q, 1, P1.p, q, 1), achr::kPrecBF16x3,
+             static_cast<int>(Msamp));
+    // Gd (df x q, row major) = Y^T P1
+    GEMM_TRY(gemm(df, q, static_cast<int>(Msamp), Y.p, 1, df, P1.p, q, 1, Gd.p, q, 1), achr::kPrecBF16x3, df);
+    std::vector<float> Gh(static_cast<size_t>(df) * q);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(Gh.data(), Gd.p, sizeof(float) * Gh.size(), hipMemcpyDeviceToHost));
+    std::vector<double> G(static_cast<size_t>(df) * q);  // column major
+    for (int i = 0; i < df; ++i)
+      for (int j = 0; j < q; ++j) G[static_cast<size_t>(i) + static_cast<size_t>(j) * df] = Gh[static_cast<size_t>(i) * q + j];
+    if (!chol_qr(G, df, q)) return fail(ACHR_E_SOLVER, "range finder basis is rank deficient");
+    std::vector<float> Qh(static_cast<size_t>(df) * q);
+    for (size_t i = 0; i < Qh.size(); ++i) Qh[i] = static_cast<float>(G[i]);
+    DevBuf<float> Qq;
+    HIP_TRY(Qq.upload(Qh.data(), Qh.size(), nullptr));
+    // Bq = Y Q (M x q), Cq = Bq^T Bq / M (q x q); keep the top-r eigen-directions of Cq.
+    DevBuf<float> Bq, Cq;
+    HIP_TRY(Bq.ensure(static_cast<size_t>(Msamp) * q));
+    HIP_TRY(Cq.ensure(static_cast<size_t>(q) * q));
+    GEMM_TRY(gemm(static_cast<int>(Msamp), q, df, Y.p, df, 1, Qq.p, 1, df, Bq.p, q, 1), achr::kPrecBF16x3,
+             static_cast<int>(Msamp));
+    GEMM_TRY(gemm(q, q, static_cast<int>(Msamp), Bq.p, 1, q, Bq.p, q, 1, Cq.p, q, 1,
+                  1.0f / static_cast<float>(Msamp)),
+             achr::kPrecBF16x3, q);
+    // total variance |Y|_F^2 / M
+    std::vector<float> Cqh(static_cast<size_t>(q) * q);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(Cqh.data(), Cq.p, sizeof(float) * Cqh.size(), hipMemcpyDeviceToHost));
+    {
+      double tot = 0.0;
+      HIP_TRY(achr::sum_squares(Y.p, static_cast<size_t>(Msamp) * df, &tot));
+      tot /= static_cast<double>(Msamp);
+      // symmetric eigen-decomposition of Cq (q x q, small): Jacobi on the host
+      std::vector<double> A(static_cast<size_t>(q) * q), V(static_cast<size_t>(q) * q, 0.0);
+      for (int i = 0; i < q; ++i)
+        for (int j = 0; j < q; ++j)
+          A[i + static_cast<size_t>(j) * q] = 0.5 * (Cqh[static_cast<size_t>(i) * q + j] + Cqh[static_cast<size_t>(j) * q + i]);
+      for (int i = 0; i < q; ++i) V[i + static_cast<size_t>(i) * q] = 1.0;
+      for (int sweep = 0; sweep < 30; ++sweep) {
+        double off = 0.0;
+        for (int j = 0; j < q; ++j)
+          for (int i = 0; i < j; ++i) off += A[i + static_cast<size_t>(j) * q] * A[i + static_cast<size_t>(j) * q];
+        if (off < 1e-22) break;
+        for (int p = 0; p < q - 1; ++p)
+          for (int qq = p + 1; qq < q; ++qq) {
+            const double apq = A[p + static_cast<size_t>(qq) * q];
+            if (std::fabs(apq) < 1e-300) continue;
+            const double th = (A[qq + static_cast<size_t>(qq) * q] - A[p + static_cast<size_t>(p) * q]) / (2.0 * apq);
+            const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+            const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+            for (int k = 0; k < q; ++k) {
+              const double akp = A[k + static_cast<size_t>(p) * q], akq = A[k + static_cast<size_t>(qq) * q];
+              A[k + static_cast<size_t>(p) * q] = c * akp - s * akq;
+              A[k + static_cast<size_t>(qq) * q] = s * akp + c * akq;
+            }
+            for (int k = 0; k < q; ++k) {
+              const double apk = A[p + static_cast<size_t>(k) * q], aqk = A[qq + static_cast<size_t>(k) * q];
+              A[p + static_cast<size_t>(k) * q] = c * apk - s * aqk;
+              A[qq + static_cast<size_t>(k) * q] = s * apk + c * aqk;
+            }
+            for (int k = 0; k < q; ++k) {
+              const double vkp = V[k + static_cast<size_t>(p) * q], vkq = V[k + static_cast<size_t>(qq) * q];
+              V[k + static_cast<size_t>(p) * q] = c * vkp - s * vkq;
+              V[k + static_cast<size_t>(qq) * q] = s * vkp + c * vkq;
+            }
+          }
+      }
+      std::vector<int> ord(q);
+      for (int i = 0; i < q; ++i) ord[i] = i;
+      std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+        return A[a + static_cast<size_t>(a) * q] > A[b + static_cast<size_t>(b) * q];
+      });
+      // Q_r = Q V[:, top r]; target C_r = diag(top r eigenvalues)
+      std::vector<float> Qr(static_cast<size_t>(df) * r, 0.0f);
+      std::vector<float> Cr(static_cast<size_t>(r) * r, 0.0f);
+      captured = 0.0;
+      for (int j = 0; j < r; ++j) {
+        const int src = ord[j];
+        const double lam = std::max(0.0, A[src + static_cast<size_t>(src) * q]);
+        Cr[static_cast<size_t>(j) * r + j] = static_cast<float>(lam);
+        captured += lam;
+        for (int i = 0; i < df; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < q; ++k) s += G[static_cast<size_t>(i) + static_cast<size_t>(k) * df] * V[k + static_cast<size_t>(src) * q];
+          Qr[static_cast<size_t>(i) + static_cast<size_t>(j) * df] = static_cast<float>(s);
+        }
+      }
+      sig2 = df > r ? std::max(tot - captured, 0.0) / (df - r) : 0.0;
+      captured = tot > 0.0 ? captured / tot : 0.0;
+      // The network models the whitened law in these axes, scaled so the
+      // target has unit mean eigenvalue.
+      HIP_TRY(Q32.upload(Qr.data(), Qr.size(), nullptr));
+      double mean_lam = 0.0;
+      for (int j = 0; j < r; ++j) mean_lam += Cr[static_cast<size_t>(j) * r + j];
+      mean_lam /= r;
+      const double scale = mean_lam > 0.0 ? 1.0 / mean_lam : 1.0;
+      for (auto& v : Cr) v = static_cast<float>(v * scale);
+      sig2 *= scale;
+      // Train the network.
+      achr_dirnet* net = nullptr;
+      rc = achr_dirnet_create(r, o.width, r, seed, &net);
+      if (rc != ACHR_OK) return rc;
+      rc = achr_dirnet_set_target(net, Cr.data());
+      const double tt = now_ms();
+      std::vector<float> losses(std::max(o.iters, 1));
+      if (rc == ACHR_OK) rc = achr_dirnet_train(net, o.iters, o.batch, o.lr, seed, losses.data());
+      train_ms = now_ms() - tt;
+      if (rc == ACHR_OK && o.iters > 0) {
+        loss0 = losses.front();
+        loss1 = losses[o.iters - 1];
+      }
+      // Bank coordinates: Wmix (K x df) = sigma (G - (G Q) Q^T) + V Q^T, V = v(z) (BF16 inference).
+      if (rc == ACHR_OK) {
+        achr::DirNet& nn = net->net;
+        DevBuf<float> Vb, GQ;
+        HIP_TRY(Vb.ensure(static_cast<size_t>(K) * r));
+        HIP_TRY(GQ.ensure(static_cast<size_t>(K) * r));
+        const int chunk = 4096;
+        for (int k0 = 0; k0 < K; k0 += chunk) {
+          const int cnt = std::min(chunk, K - k0);
+          HIP_TRY(nn.ensure_batch(cnt));
+          HIP_TRY(nn.gaussian_latents(cnt, seed ^ 0x5bd1e995u, static_cast<uint32_t>(0x10000 + k0), nullptr));
+          HIP_TRY(nn.forward(cnt, achr::kPrecBF16, nullptr));
+          HIP_TRY(hipMemcpyAsync(Vb.p + static_cast<size_t>(k0) * r, nn.v, sizeof(float) * cnt * r,
+                                 hipMemcpyDeviceToDevice, nullptr));
+        }
+        const float sig = static_cast<float>(std::sqrt(sig2));
+        HIP_TRY(achr::launch_gauss(Wmix.p, static_cast<size_t>(K) * df, seed, 0xB2u, nullptr));
+        // GQ = G Q (K x r)
+        GEMM_TRY(gemm(K, r, df, Wmix.p, df, 1, Q32.p, 1, df, GQ.p, r, 1), achr::kPrecBF16x3, K);
+        // GQ <- V - sigma GQ, then Wmix <- sigma Wmix + GQ Q^T
+        HIP_TRY(achr::launch_axpby(GQ.p, Vb.p, -sig, 1.0f, static_cast<size_t>(K) * r, nullptr));
+        // C(k, i) with B(j, i) = Q[i + j*df]
+        GEMM_TRY(gemm(K, df, r, GQ.p, r, 1, Q32.p, df, 1, Wmix.p, df, 1, 1.0f, sig), achr::kPrecBF16x3, K);
+      }
+      achr_dirnet_destroy(net);
+      if (rc != ACHR_OK) return rc;
+    }
+  }
+  // Bank (n x K column major, FP64) = N_f Wmix^T, then unit columns.
+  DevBuf<double> W64;
+  HIP_TRY(W64.ensure(static_cast<size_t>(K) * df));
+  HIP_TRY(achr::launch_to_f64(Wmix.p, W64.p, static_cast<size_t>(K) * df, nullptr));
+  HIP_TRY(S->dirs.ensure(static_cast<size_t>(n) * K));
+  DevBuf<int> kcount;
+  HIP_TRY(kcount.upload(&K, 1, nullptr));
+  HIP_TRY(achr::launch_dgemm_cols(n, df, Nf.p, 1, n, W64.p, df, S->dirs.p, n, kcount.p, K, nullptr));
+  HIP_TRY(achr::launch_normalize_cols(S->dirs.p, n, K, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  S->K = K;
+  S->use_dirs = 1;
+  if (report) {
+    report[0] = df;
+    report[1] = static_cast<double>(Msamp);
+    report[2] = loss0;
+    report[3] = loss1;
+    report[4] = sig2;
+    report[5] = captured;
+    report[6] = now_ms() - t0;
+    report[7] = train_ms;
+  }
+  return ACHR_OK;
+}
+
+}  // extern "C"

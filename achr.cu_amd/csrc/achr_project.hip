@@ -14,6 +14,7 @@
 #include <algorithm>
 
 #include "achr_internal.h"
+#include "achr_philox.h"
 
 namespace achr {
 namespace {
@@ -143,7 +144,140 @@ __global__ void to_f32_kernel(const double* __restrict__ src, float* __restrict_
     dst[i] = static_cast<float>(src[i]);
 }
 
+__global__ __launch_bounds__(256) void normalize_cols_kernel(double* __restrict__ A, int n) {
+  double* col = A + static_cast<size_t>(blockIdx.x) * n;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s = fma(col[i], col[i], s);
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double nrm = sqrt(red[0]);
+  if (nrm > 0.0)
+    for (int i = threadIdx.x; i < n; i += 256) col[i] = col[i] / nrm;
+}
+
+__global__ void col_means_kernel(const double* __restrict__ X, long rows, int cols, double* __restrict__ mean) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double s = 0.0;
+  for (long r = 0; r < rows; ++r) s += X[r * cols + j];
+  mean[j] = s / static_cast<double>(rows);
+}
+
+__global__ void sub_row_kernel(float* __restrict__ Y, long rows, int cols, const float* __restrict__ v) {
+  const size_t n = static_cast<size_t>(rows) * cols;
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    Y[i] -= v[i % cols];
+}
+
+__global__ void gauss_fill_kernel(float* __restrict__ z, size_t n, uint64_t seed, uint32_t sid) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const u32x4 r = stream_block(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), sid, 0u, kTagDir);
+    const double u1 = (static_cast<double>(r.v0) + 1.0) * 0x1.0p-32;
+    const double u2 = static_cast<double>(r.v1) * 0x1.0p-32;
+    z[i] = static_cast<float>(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+  }
+}
+
+__global__ void axpby_kernel(float* __restrict__ y, const float* __restrict__ x, float alpha, float beta, size_t n) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    y[i] = alpha * y[i] + beta * x[i];
+}
+
+__global__ void to_f64_kernel(const float* __restrict__ src, double* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    dst[i] = static_cast<double>(src[i]);
+}
+
+__global__ __launch_bounds__(256) void sumsq_kernel(const float* __restrict__ x, size_t n, double* __restrict__ part) {
+  double s = 0.0;
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    s += static_cast<double>(x[i]) * x[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+unsigned blocks_for(size_t n) {
+  size_t b = (n + 255) / 256;
+  return static_cast<unsigned>(b > 16384 ? 16384 : (b == 0 ? 1 : b));
+}
+
 }  // namespace
+
+hipError_t launch_dgemm_cols(int M, int K, const double* A, long sar, long sak, const double* B, long ldb,
+                             double* C, long ldc, const int* count, int max_cols, hipStream_t stream) {
+  const unsigned gq = static_cast<unsigned>((max_cols + BN - 1) / BN);
+  hipLaunchKernelGGL(dgemm_seq_kernel, dim3((M + BM - 1) / BM, gq), dim3(kThreads), 0, stream, M, K, A, sar, sak, B,
+                     ldb, static_cast<const int*>(nullptr), C, ldc, static_cast<const int*>(nullptr), count);
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize_cols(double* A, int n, int K, hipStream_t stream) {
+  if (K <= 0) return hipSuccess;
+  hipLaunchKernelGGL(normalize_cols_kernel, dim3(K), dim3(256), 0, stream, A, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_col_means(const double* X, long rows, int cols, double* mean, hipStream_t stream) {
+  hipLaunchKernelGGL(col_means_kernel, dim3((cols + 255) / 256), dim3(256), 0, stream, X, rows, cols, mean);
+  return hipGetLastError();
+}
+
+hipError_t launch_sub_row(float* Y, long rows, int cols, const float* v, hipStream_t stream) {
+  hipLaunchKernelGGL(sub_row_kernel, dim3(blocks_for(static_cast<size_t>(rows) * cols)), dim3(256), 0, stream, Y,
+                     rows, cols, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_gauss(float* z, size_t count, uint64_t seed, uint32_t sid, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(gauss_fill_kernel, dim3(blocks_for(count)), dim3(256), 0, stream, z, count, seed, sid);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby(float* y, const float* x, float alpha, float beta, size_t count, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(axpby_kernel, dim3(blocks_for(count)), dim3(256), 0, stream, y, x, alpha, beta, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_f64(const float* src, double* dst, size_t count, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(to_f64_kernel, dim3(blocks_for(count)), dim3(256), 0, stream, src, dst, count);
+  return hipGetLastError();
+}
+
+hipError_t sum_squares(const float* x, size_t count, double* out) {
+  constexpr unsigned kBlocks = 1024;
+  double* part = nullptr;
+  hipError_t e = hipMalloc(&part, sizeof(double) * kBlocks);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sumsq_kernel, dim3(kBlocks), dim3(256), 0, nullptr, x, count, part);
+  e = hipGetLastError();
+  double h[kBlocks];
+  if (e == hipSuccess) e = hipMemcpy(h, part, sizeof h, hipMemcpyDeviceToHost);
+  (void)hipFree(part);
+  if (e != hipSuccess) return e;
+  double s = 0.0;
+  for (unsigned i = 0; i < kBlocks; ++i) s += h[i];
+  *out = s;
+  return hipSuccess;
+}
 
 hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_t stream) {
   if (count == 0) return hipSuccess;

@@ -167,7 +167,7 @@ struct Row {
   __device__ __forceinline__ void st(int i, double v) const { bstore(r, off(i), v); }
 };
 
-template <int TPC, int E, int CPB>
+template <int TPC, int E, int CPB, bool DIR>
 __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
   extern __shared__ double lds[];
   double* red = lds;  // 4 reduction slot arrays; only block-per-chain schedules use them
@@ -189,10 +189,11 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
   const Row<TPC> trow(a.T, b, d, d);   // N^T x of the in-kernel projection
 
   // Large slot counts keep the running centre in LDS (its own slots only, so
-  // no barrier is needed); x and the direction stay in registers.
-  constexpr bool kCL = E > 8;
+  // no barrier is needed); x and the direction stay in registers. With a
+  // direction bank (DIR) there is no centre.
+  constexpr bool kCL = !DIR && E > 8;
   double x[E];
-  double c[kCL ? 1 : E];
+  double c[(DIR || kCL) ? 1 : E];
   double* cl = lds + 4 * kRedSlots + static_cast<size_t>(g) * (TPC * E);  // this chain's slice
   if (st.z == kFresh) {
     const rsrc_t r_c0 = make_rsrc(a.c0, 8LL * n);
@@ -200,15 +201,22 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
     for (int k = 0; k < E; ++k) {
       const double v = bload(r_c0, (lane + k * TPC) * 8);
       x[k] = v;
-      if constexpr (kCL) cl[lane + k * TPC] = v; else c[k] = v;
+      if constexpr (DIR) {
+      } else if constexpr (kCL) {
+        cl[lane + k * TPC] = v;
+      } else {
+        c[k] = v;
+      }
     }
     st = make_int4(0, 0, kRunning, 0);
   } else {
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       x[k] = xrow.ld(lane + k * TPC);
-      const double cv = crow.ld(lane + k * TPC);
-      if constexpr (kCL) cl[lane + k * TPC] = cv; else c[k] = cv;
+      if constexpr (!DIR) {
+        const double cv = crow.ld(lane + k * TPC);
+        if constexpr (kCL) cl[lane + k * TPC] = cv; else c[k] = cv;
+      }
     }
   }
 
@@ -223,21 +231,52 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
   bool stopped_at_check = false;
   const uint32_t chain = static_cast<uint32_t>(a.chain0 + b);
 
+  // Bank mode: the direction of attempt a + 1 is loaded while attempt a runs
+  // (the pick is a function of the attempt counter only).
+  double wn[DIR ? E : 1];
+  auto load_dir = [&](uint32_t att) {
+    const int idn = pick_index(stream_u01(a.seed, att, chain, a.file, a.model, kTagPick), a.W);
+    if constexpr (TPC >= 64) {
+      const int idu = __builtin_amdgcn_readfirstlane(idn);
+      const rsrc_t r_w = make_rsrc(a.warm + static_cast<size_t>(idu) * n, 8LL * n);
+#pragma unroll
+      for (int k = 0; k < E; ++k) wn[k] = bload(r_w, (lane + k * TPC) * 8);
+    } else {
+      const int wbase = idn * n;
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int i = lane + k * TPC;
+        wn[k] = bload(r_wall, i < n ? (wbase + i) * 8 : kOob);
+      }
+    }
+  };
+  if constexpr (DIR) load_dir(attempt);
+
   for (;;) {
     if (!pending) {
       if (s >= a.S) break;
-      const double up = stream_u01(a.seed, attempt, chain, a.file, a.model, kTagPick);
-      ++attempt;
-      const int id = pick_index(up, a.W);
       double u[E];
       double ss = 0.0;
-      if constexpr (TPC >= 64) {
+      int id = 0;
+      if constexpr (DIR) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) u[k] = wn[k];
+        ++attempt;
+        load_dir(attempt);
+      } else {
+        const double up = stream_u01(a.seed, attempt, chain, a.file, a.model, kTagPick);
+        ++attempt;
+        id = pick_index(up, a.W);
+      }
+      if constexpr (DIR) {
+      } else if constexpr (TPC >= 64) {
         // id is uniform over the wavefront: a per-column resource in SGPRs.
         const int idu = __builtin_amdgcn_readfirstlane(id);
         const rsrc_t r_w = make_rsrc(a.warm + static_cast<size_t>(idu) * n, 8LL * n);
 #pragma unroll
         for (int k = 0; k < E; ++k) {
-          u[k] = bload(r_w, (lane + k * TPC) * 8) - (kCL ? cl[lane + k * TPC] : c[k]);
+          const double wk = bload(r_w, (lane + k * TPC) * 8);
+          u[k] = wk - (kCL ? cl[lane + k * TPC] : c[k]);
           ss = fma(u[k], u[k], ss);
         }
       } else {
@@ -245,17 +284,22 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < E; ++k) {
           const int i = lane + k * TPC;
-          u[k] = bload(r_wall, i < n ? (wbase + i) * 8 : kOob) - (kCL ? cl[i] : c[k]);
+          const double wk = bload(r_wall, i < n ? (wbase + i) * 8 : kOob);
+          u[k] = wk - (kCL ? cl[i] : c[k]);
           ss = fma(u[k], u[k], ss);
         }
       }
-      ss = group_sum<TPC>(ss, red);
-      const double nrm = sqrt(ss);
+      // Bank directions are unit vectors: no norm, no division.
+      double nrm = 1.0;
+      if constexpr (!DIR) {
+        ss = group_sum<TPC>(ss, red);
+        nrm = sqrt(ss);
+      }
       double mn = -INFINITY, mx = INFINITY;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const int off = (lane + k * TPC) * 8;
-        const double ui = u[k] / nrm;
+        const double ui = DIR ? u[k] : u[k] / nrm;
         u[k] = ui;
         const double du = bload(r_ub, off) - x[k];
         const double dl = x[k] - bload(r_lb, off);
@@ -347,7 +391,9 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
       const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
       if (x[k] > ubi) x[k] = ubi;
       else if (x[k] < lbi) x[k] = lbi;
-      if constexpr (kCL) {
+      if constexpr (DIR) {
+        // fixed direction law: no centre
+      } else if constexpr (kCL) {
         cl[lane + k * TPC] = fma(alpha, cl[lane + k * TPC], beta * x[k]);
       } else {
         c[k] = fma(alpha, c[k], beta * x[k]);
@@ -367,7 +413,8 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
 
   if (stopped_at_check) {
 #pragma unroll
-    for (int k = 0; k < E; ++k) crow.st(lane + k * TPC, kCL ? cl[lane + k * TPC] : c[k]);
+    for (int k = 0; k < E; ++k)
+      if constexpr (!DIR) crow.st(lane + k * TPC, kCL ? cl[lane + k * TPC] : c[k]);
     st = make_int4(static_cast<int>(attempt), s, kPendingCheck, consec);
   } else {
 #pragma unroll
@@ -389,11 +436,14 @@ hipError_t launch_tpc_e(const StepArgs& a, hipStream_t stream, size_t lds) {
   const unsigned grid = static_cast<unsigned>((a.P + CPB - 1) / CPB);
   if (lds > 65536) {
     // Dynamic LDS beyond 64 KiB must be opted into per kernel.
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&step_kernel<TPC, E, CPB>),
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&step_kernel<TPC, E, CPB, false>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((step_kernel<TPC, E, CPB>), dim3(grid), dim3(TPC * CPB), lds, stream, a);
+  if (a.dir_mode)
+    hipLaunchKernelGGL((step_kernel<TPC, E, CPB, true>), dim3(grid), dim3(TPC * CPB), lds, stream, a);
+  else
+    hipLaunchKernelGGL((step_kernel<TPC, E, CPB, false>), dim3(grid), dim3(TPC * CPB), lds, stream, a);
   return hipGetLastError();
 }
 
@@ -428,9 +478,9 @@ int elems_for(int n, int tpc) {
   return -1;
 }
 
-size_t step_lds_bytes(const StepArgs&, const Schedule& s) {
+size_t step_lds_bytes(const StepArgs& a, const Schedule& s) {
   size_t bytes = 4 * kRedSlots * sizeof(double);
-  if (s.e > 8) bytes += static_cast<size_t>(s.cpb) * s.tpc * s.e * sizeof(double);  // centres in LDS
+  if (s.e > 8 && !a.dir_mode) bytes += static_cast<size_t>(s.cpb) * s.tpc * s.e * sizeof(double);  // centres in LDS
   return bytes;
 }
 

@@ -93,20 +93,89 @@ def cpu_baseline(m, W, N, P_cpu, S_cpu, threads, gpu_ess_per_step):
             "chain_steps_per_s": steps_per_s}
 
 
+def measure(smp, P, S, steps, warmup, seed0=1):
+    """Warmup files (20 steps each) then `steps` timed files; returns (wall, timing) of the slowest."""
+    for i in range(warmup):
+        smp.run(P, min(S, 20), seed=1000 + i)
+        smp.timing()
+    times, timings = [], []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        smp.run(P, S, seed=seed0 + k, file=k)
+        tm = smp.timing()  # waits for the end event on the sampler's stream
+        times.append(time.perf_counter() - t0)
+        timings.append(tm)
+        log(f"file {k}: {times[-1]:.3f}s wall, kernels {tm}")
+    j = int(np.argmax(times))
+    return times[j], timings[j]
+
+
+def sample_quality(m, smp, S, trace_stride_chains, P):
+    from achr import stats as st
+    stats_ = smp.stats()
+    pts = smp.points()
+    trace = smp.trace()
+    live = (m.ub - m.lb) > 0
+    ess = st.bulk_ess(trace[:, :, live])
+    min_ess = float(np.nanmin(ess))
+    if m.S is not None:
+        drift = float(np.abs(m.S @ pts).max())
+    else:
+        import scipy.sparse as sp
+        drift = float(np.abs(sp.csr_matrix((m.vals, (m.rows, m.cols)), shape=(m.m, m.n)) @ pts).max())
+    rhat = st.rhat(trace[:, :, live])
+    return {"stats": stats_, "pts": pts, "min_ess": min_ess, "ess_per_step": min_ess / (trace.shape[0] * S),
+            "traced": int(trace.shape[0]), "drift": drift, "max_rhat": float(np.nanmax(rhat))}
+
+
+def roofline_for(args_proj, m, n, d, P, S, q, tm):
+    reproj = int(q["stats"][:, 1].sum())
+    attempts = P * S + int(q["stats"][:, 0].sum())
+    segs = tm["step_launches"]
+    step_bytes = 8.0 * n * attempts + 32.0 * n * P * max(segs - 1, 0)
+    step_ms, proj_ms = tm["ms_step"], tm["ms_proj"]
+    if args_proj == "fp64":
+        proj_flops = 4.0 * n * d * reproj
+        kname, peak, per = "dgemm_seq_kernel (batched N (N^T X), FP64 FMA)", PEAK_FP64_TFLOPS, "4*n*d"
+    else:
+        proj_flops = reproj * (0.9 * 2.0 * n * m.m + 0.1 * 4.0 * n * d)
+        kname = f"gemm_kernel ({args_proj}: X -= S+ (S X)) + FP64 rounds"
+        peak = PEAK_BF16_TFLOPS / 3 if "bf16" in args_proj else PEAK_FP32_TFLOPS
+        per = "2*n*m (residual) / 4*n*d (FP64 round)"
+    if proj_ms > step_ms:
+        ach = proj_flops / (proj_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": kname, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                "frac": ach / peak, "traffic": None, "launches": tm["proj_launches"],
+                "per_launch": f"{per} flops per flagged chain; {reproj} chain reprojections over "
+                              f"{tm['proj_launches']} rounds"}
+    ach = step_bytes / (step_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "step_kernel", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": ach / PEAK_HBM_GBS, "traffic": None, "launches": segs,
+            "per_launch": f"8n B per attempt (direction gather) + 32n B per chain per resumed segment; "
+                          f"{attempts} attempts, {segs} launches"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed files (bench steps)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed files (short: 20 chain steps)")
     ap.add_argument("--workload", choices=["genome", "ecoli"], default="genome")
+    ap.add_argument("--sampler", choices=["learned", "plain"], default="learned")
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--chain-steps", type=int, default=None)
     ap.add_argument("--trace-chains", type=int, default=64)
     ap.add_argument("--trace-thin", type=int, default=50)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--proj", choices=list(PROJ), default="fp64",
-                    help="arithmetic of the batched projection rounds")
+                    help="arithmetic of the batched projection rounds of plain ACHR")
     ap.add_argument("--fp64-every", type=int, default=10)
+    ap.add_argument("--train-steps", type=int, default=1000, help="plain-ACHR steps that produce the training set")
+    ap.add_argument("--train-thin", type=int, default=100)
+    ap.add_argument("--rank", type=int, default=512)
+    ap.add_argument("--iters", type=int, default=400)
+    ap.add_argument("--check-every", type=int, default=100, help="residual check period of the learned sampler")
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 ACHR comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-chains", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=300)
@@ -128,112 +197,83 @@ def main():
     P = args.chains or P
     S = args.chain_steps or S
     N = m.N
+    n, d = m.n, N.shape[1]
     model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
     model.set_nullspace(N)
-    t = time.time()
-    opts = achr.Options(lanes=args.lanes, trace_chains=min(args.trace_chains, P), trace_thin=args.trace_thin,
-                        proj_prec=PROJ[args.proj], fp64_every=args.fp64_every,
-                        reproject_mode=2 if args.proj != "fp64" else 0)
-    smp = achr.Sampler(model, W, opts)
-    lanes, mode = smp.schedule()
-    log(f"sampler ready in {time.time() - t:.1f}s: {lanes} threads/chain, reprojection mode {mode}")
-
-    for i in range(args.warmup):
-        smp.run(P, min(S, 20), seed=1000 + i)
+    tc = min(args.trace_chains, P)
+    out = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    cfg = {"workload": f"{args.workload}: {m.m}x{m.n} model, {W.shape[1]} warmup points (%f), {P} chains x {S} "
+                       f"steps", "chains": P, "chain_steps": S, "reactions": n, "metabolites": m.m, "null_dim": d}
+    plain_q = None
+    if args.sampler == "plain" or not args.no_compare:
+        t = time.time()
+        opts = achr.Options(lanes=args.lanes, trace_chains=tc, trace_thin=args.trace_thin,
+                            proj_prec=PROJ[args.proj], fp64_every=args.fp64_every,
+                            reproject_mode=2 if args.proj != "fp64" else 0)
+        smp = achr.Sampler(model, W, opts)
+        lanes, mode = smp.schedule()
+        log(f"plain sampler ready in {time.time() - t:.1f}s: {lanes} threads/chain, reprojection mode {mode}")
+        wall_p, tm_p = measure(smp, P, S, args.steps if args.sampler == "plain" else 1, args.warmup)
+        plain_q = sample_quality(m, smp, S, tc, P)
+        plain_value = plain_q["min_ess"] * (P / plain_q["traced"]) / wall_p
+        plain = {"min_ess_per_s": plain_value, "s_per_file": wall_p, "chain_steps_per_s": P * S / wall_p,
+                 "min_ess_traced": plain_q["min_ess"], "drift_inf": plain_q["drift"],
+                 "max_rhat": plain_q["max_rhat"], "kernel_ms": tm_p, "projection": args.proj}
+        log(f"plain: {plain}")
+        del smp
+    if args.sampler == "plain":
+        out.update({"metric": f"min-ESS/sec over reactions (plain ACHR, projection {args.proj})",
+                    "value": plain_value, "unit": "min-ESS/s", "ms_per_step": wall_p * 1e3})
+        cfg.update({"sampler": "plain ACHR", "lanes": lanes, "reproject_mode": mode})
+        out["config"] = cfg
+        out["roofline"] = roofline_for(args.proj, m, n, d, P, S, plain_q, tm_p)
+        out.update({k: v for k, v in plain.items() if k != "kernel_ms"})
+        ess_per_step_ref = plain_q["ess_per_step"]
+    else:
+        # 1. training set: plain ACHR (residual BF16x3 projection) with every chain traced
+        t = time.time()
+        topts = achr.Options(lanes=args.lanes, trace_chains=P, trace_thin=args.train_thin,
+                             proj_prec=PROJ["res_bf16x3"], reproject_mode=2)
+        smp = achr.Sampler(model, W, topts)
+        smp.run(P, args.train_steps, seed=99)
         smp.timing()
-        log(f"warmup file {i} done")
-    times, timings = [], []
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        smp.run(P, S, seed=1 + k, file=k)
-        tm = smp.timing()  # waits for the end event on the sampler's stream
-        times.append(time.perf_counter() - t0)
-        timings.append(tm)
-        log(f"file {k}: {times[-1]:.3f}s wall, kernels {tm}")
-    wall = max(times)
-    tm = timings[int(np.argmax(times))]
-    stats_ = smp.stats()
-    pts = smp.points()
-    trace = smp.trace()
-    # ESS over the traced chains (skip fixed reactions: zero range, no variance)
-    live = (m.ub - m.lb) > 0
-    ess = st.bulk_ess(trace[:, :, live])
-    min_ess = float(np.nanmin(ess))
-    ess_per_step = min_ess / (trace.shape[0] * S)
-    value = min_ess * (P / trace.shape[0]) / wall
-    Sd = m.dense() if m.S is not None else None
-    if Sd is not None:
-        drift = float(np.abs(Sd @ pts).max())
-    else:
-        import scipy.sparse as sp
-        Ssp = sp.csr_matrix((m.vals, (m.rows, m.cols)), shape=(m.m, m.n))
-        drift = float(np.abs(Ssp @ pts).max())
-    half = P // 2
-    em, es = st.moment_error(pts[:, :half], pts[:, half:].mean(axis=1), pts[:, half:].std(axis=1, ddof=1), m.lb,
-                             m.ub)
-    accepted = P * S
-    attempts = accepted + int(stats_[:, 0].sum())
-    reproj = int(stats_[:, 1].sum())
-    segs = tm["step_launches"]
-    n, d = m.n, N.shape[1]
-    step_bytes = 8.0 * n * attempts + 32.0 * n * P * max(segs - 1, 0)
-    step_ms, proj_ms = tm["ms_step"], tm["ms_proj"]
-    if args.proj == "fp64":
-        proj_flops = 4.0 * n * d * reproj
-        kname, peak, per = "dgemm_seq_kernel (batched N (N^T X), FP64 FMA)", PEAK_FP64_TFLOPS, "4*n*d"
-    elif args.proj.startswith("res"):
-        # rounds mix residual contractions (2*n*m per chain) and FP64 rounds (4*n*d per chain)
-        low_frac = 1.0 - 1.0 / args.fp64_every
-        proj_flops = reproj * (low_frac * 2.0 * n * m.m + (1 - low_frac) * 4.0 * n * d)
-        kname = f"gemm_kernel ({args.proj}: X -= S+ (S X)) + FP64 rounds"
-        peak = PEAK_BF16_TFLOPS / 3 if "bf16" in args.proj else PEAK_FP32_TFLOPS
-        per = "2*n*m (residual) / 4*n*d (FP64 round)"
-    else:
-        proj_flops = 4.0 * n * d * reproj
-        kname = f"gemm_kernel ({args.proj}: N (N^T X))"
-        peak = PEAK_BF16_TFLOPS / 3 if "bf16" in args.proj else PEAK_FP32_TFLOPS
-        per = "4*n*d"
-    if proj_ms > step_ms:
-        ach = proj_flops / (proj_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": kname, "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None, "launches": tm["proj_launches"],
-                "per_launch": f"{per} flops per flagged chain; {reproj} chain reprojections over "
-                              f"{tm['proj_launches']} rounds"}
-    else:
-        ach = step_bytes / (step_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": "step_kernel", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": ach / PEAK_HBM_GBS, "traffic": None, "launches": segs,
-                "per_launch": "8n B per attempt (warmup gather) + 32n B per chain per resumed segment"}
-    out = {
-        "metric": f"min-ESS/sec over reactions (plain ACHR, projection {args.proj})",
-        "value": value,
-        "unit": "min-ESS/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {m.m}x{m.n} model, {W.shape[1]} warmup points (%f), "
-                               f"{P} chains x {S} steps", "chains": P, "chain_steps": S, "reactions": n,
-                   "metabolites": m.m, "null_dim": d, "lanes": lanes, "reproject_mode": mode},
-        "chain_steps_per_s": accepted / wall,
-        "min_ess_traced": min_ess,
-        "traced_chains": int(trace.shape[0]),
-        "drift_inf": drift,
-        "moment_err_mean_frac": em,
-        "moment_err_std_frac": es,
-        "reprojections": reproj,
-        "misses": int(stats_[:, 0].sum()),
-        "kernel_ms": {"step": step_ms, "projection": proj_ms, "total_events": tm["ms_total"]},
-        "roofline": roof,
-    }
-    if not args.no_cpu:
+        t_train_run = time.time() - t
+        # 2. rounding map + network + direction bank
+        t = time.time()
+        rep = smp.learn_directions(rank=args.rank, width=512, iters=args.iters, batch=2048, lr=1e-3, seed=7)
+        t_learn = time.time() - t
+        log(f"training run {t_train_run:.1f}s, learn {t_learn:.1f}s: {rep}")
+        # 3. timed learned run
+        smp.set_run_options(tc, args.trace_thin, args.check_every)
+        lanes, mode = smp.schedule()
+        wall, tm = measure(smp, P, S, args.steps, args.warmup)
+        q = sample_quality(m, smp, S, tc, P)
+        value = q["min_ess"] * (P / q["traced"]) / wall
+        out.update({"metric": "min-ESS/sec over reactions (learned+batched vs plain ACHR)", "value": value,
+                    "unit": "min-ESS/s", "ms_per_step": wall * 1e3})
+        cfg.update({"sampler": "learned direction bank (3-layer MLP rounding map) + batched chains",
+                    "lanes": lanes, "reproject_mode": mode, "check_every": args.check_every,
+                    "bank_size": 2 * n, "rank": args.rank})
+        out["config"] = cfg
+        out.update({"chain_steps_per_s": P * S / wall, "min_ess_traced": q["min_ess"], "traced_chains": q["traced"],
+                    "drift_inf": q["drift"], "max_rhat": q["max_rhat"],
+                    "learn": {"train_run_s": t_train_run, "learn_s": t_learn, **rep},
+                    "kernel_ms": tm})
+        out["roofline"] = roofline_for("fp64", m, n, d, P, S, q, tm)
+        if plain_q is not None:
+            rng = m.ub - m.lb
+            live = rng > 0
+            dm = np.abs(q["pts"].mean(1) - plain_q["pts"].mean(1))[live] / rng[live]
+            ds = np.abs(q["pts"].std(1) - plain_q["pts"].std(1))[live] / rng[live]
+            out["plain_fp64"] = plain
+            out["speedup_vs_plain"] = value / plain["min_ess_per_s"]
+            out["moment_err_vs_plain"] = {"mean_frac_max": float(dm.max()), "std_frac_max": float(ds.max()),
+                                          "mean_frac_median": float(np.median(dm))}
+        ess_per_step_ref = plain_q["ess_per_step"] if plain_q is not None else None
+    if not args.no_cpu and ess_per_step_ref is not None:
         threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(m, W, N, args.cpu_chains, args.cpu_steps, threads, ess_per_step)
+        out["cpu_baseline"] = cpu_baseline(m, W, N, args.cpu_chains, args.cpu_steps, threads, ess_per_step_ref)
     print(json.dumps(out), flush=True)
 
 

@@ -183,6 +183,68 @@ int achr_sample_timing(achr_sampler *sampler, double *ms_total, double *ms_step,
  * reprojection mode (1 in-kernel, 2 batched). */
 int achr_sampler_schedule(const achr_sampler *sampler, int *lanes, int *reproject_mode);
 
+/* ---------------------------------------- learned direction distribution */
+
+/* Hit-and-run along a fixed bank of unit directions, each in the free null
+ * space (S b = 0, b_i = 0 for fixed reactions), drawn uniformly: a symmetric,
+ * state-independent direction law, so the uniform law on the polytope stays
+ * stationary. It replaces ACHR's warmup-minus-centre direction (and its
+ * adaptive, non-Markovian centre); bounds, chord, step, clip and the periodic
+ * FP64 residual check are those of the reference loop. No counterpart in the
+ * reference: it is the learned sampler of this library. */
+
+typedef struct achr_dirnet achr_dirnet;
+
+/* 3-layer MLP latent -> width -> width -> outdim (GELU), odd by
+ * construction: v(z) = F(z) - F(-z). FP32 parameters, Philox-initialised. */
+int achr_dirnet_create(int latent, int width, int outdim, uint64_t seed, achr_dirnet **out);
+void achr_dirnet_destroy(achr_dirnet *net);
+int achr_dirnet_param_count(const achr_dirnet *net, long *count);
+int achr_dirnet_get_params(const achr_dirnet *net, float *params);
+int achr_dirnet_set_params(achr_dirnet *net, const float *params);
+/* Target second moment C (outdim x outdim, row major). */
+int achr_dirnet_set_target(achr_dirnet *net, const float *C);
+/* Loss |V^T V / B - C|_F^2 / |C|_F^2 on the given latents (B x latent) and
+ * its gradient with respect to every parameter (param_count floats). */
+int achr_dirnet_loss_grad(achr_dirnet *net, const float *Z, int B, float *loss, float *grads);
+/* Adam on fresh Gaussian latents (Philox); losses[it] if not NULL. */
+int achr_dirnet_train(achr_dirnet *net, int iters, int batch, float lr, uint64_t seed, float *losses);
+/* V = v(Z) for B latents; prec 0 FP32, 2 BF16 matrix cores. */
+int achr_dirnet_sample(achr_dirnet *net, const float *Z, int B, int prec, float *V);
+
+typedef struct achr_learn_options {
+  int rank;         /* principal subspace the network models (e.g. 512) */
+  int width;        /* hidden width (512) */
+  int iters;        /* Adam iterations */
+  int batch;        /* latents per iteration */
+  float lr;
+  int bank_size;    /* directions in the bank */
+  int isotropic;    /* 1: no network, isotropic directions in the free null space */
+  uint64_t seed;
+} achr_learn_options;
+
+void achr_learn_options_default(achr_learn_options *opts);
+
+/* Builds the direction bank from the samples of the last run's trace
+ * (trace_chains x S/trace_thin points): their covariance in an orthonormal
+ * basis of the free null space, reduced to its top `rank` principal
+ * directions (randomised range finder), is the network's target; bank
+ * directions are N_f (Q v + sigma_perp (I - Q Q^T) g), v from the trained
+ * network, g Gaussian, assembled in FP64 and normalised. report (8 doubles):
+ * free null dimension, samples used, loss before and after training, sigma_perp^2,
+ * captured variance fraction, setup ms, training ms. Then switches the sampler to
+ * bank directions (achr_sampler_use_directions(s, 1)). */
+int achr_sampler_learn_directions(achr_sampler *sampler, const achr_learn_options *opts, double *report);
+
+/* Changes the per-run settings of an existing sampler: trace_chains,
+ * trace_thin (0, 0 = no trace) and check_every (residual check period). */
+int achr_sampler_set_run_options(achr_sampler *sampler, int trace_chains, int trace_thin, int check_every);
+
+/* Supplies the bank directly (n x K column major, host array). */
+int achr_sampler_set_directions(achr_sampler *sampler, const double *bank, int K);
+/* 1: bank directions; 0: ACHR directions (the reference's). */
+int achr_sampler_use_directions(achr_sampler *sampler, int on);
+
 /* -------------------------------------------------------------- file I/O */
 
 /* Writes File<k>-style text: n lines of P comma-separated "%f" values

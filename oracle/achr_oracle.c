@@ -40,6 +40,11 @@
  *   - dot products of the projection N (N^T x) and of the residual S x are
  *     sequential in the reference's loop order (ACHRCuda.cu:214-216, :226-228).
  *
+ * Bank mode (oracle_sample with a bank): the library's learned sampler —
+ * the direction of an attempt is a stored unit column of the bank (picked
+ * like a warmup column), with no normalisation and no centre; everything
+ * else is the loop above. No counterpart in the reference.
+ *
  * Deliberate differences from the reference (all documented behaviour of the
  * new sampler, not approximations):
  *   - independent uniforms for the warmup pick and the step length (the
@@ -205,6 +210,8 @@ typedef struct {
   const int *rowptr, *colidx; /* CSR of S, m rows, columns ascending */
   const double *vals;
   const double *N;      /* n x d column major, may be NULL when d == 0 */
+  const double *bank;   /* optional: n x K unit directions (bank mode) */
+  int K;
 } oracle_model;
 
 typedef struct {
@@ -233,18 +240,25 @@ static void run_chain(const oracle_model *M, const oracle_opts *o, int P, int S,
   while (s < S) {
     double up = stream_u01(o->seed, attempt, chain, o->file, o->model, 0u);
     attempt++;
-    int id = (int)((double)M->W * up);
-    if (id >= M->W) id = M->W - 1;
-    const double *w = M->warm + (size_t)id * n;
-    for (int l = 0; l < o->lanes; ++l) lanes[l] = 0.0;
-    for (int i = 0; i < n; ++i) {
-      u[i] = w[i] - c[i];
-      lanes[i % o->lanes] = fma(u[i], u[i], lanes[i % o->lanes]);
+    const int npick = M->bank ? M->K : M->W;
+    int id = (int)((double)npick * up);
+    if (id >= npick) id = npick - 1;
+    double nrm = 1.0;
+    if (M->bank) {
+      /* bank mode: a stored unit direction, no centre */
+      memcpy(u, M->bank + (size_t)id * n, (size_t)n * sizeof(double));
+    } else {
+      const double *w = M->warm + (size_t)id * n;
+      for (int l = 0; l < o->lanes; ++l) lanes[l] = 0.0;
+      for (int i = 0; i < n; ++i) {
+        u[i] = w[i] - c[i];
+        lanes[i % o->lanes] = fma(u[i], u[i], lanes[i % o->lanes]);
+      }
+      nrm = sqrt(canon_tree(lanes, o->lanes));
     }
-    const double nrm = sqrt(canon_tree(lanes, o->lanes));
     double mn = -INFINITY, mx = INFINITY;
     for (int i = 0; i < n; ++i) {
-      const double ui = u[i] / nrm;
+      const double ui = M->bank ? u[i] : u[i] / nrm;
       u[i] = ui;
       const double du = M->ub[i] - x[i], dl = x[i] - M->lb[i];
       if (du > o->dTol && dl > o->dTol) {
@@ -298,7 +312,7 @@ static void run_chain(const oracle_model *M, const oracle_opts *o, int P, int S,
     for (int i = 0; i < n; ++i) {
       if (x[i] > M->ub[i]) x[i] = M->ub[i];
       else if (x[i] < M->lb[i]) x[i] = M->lb[i];
-      c[i] = fma(alpha, c[i], beta * x[i]);
+      if (!M->bank) c[i] = fma(alpha, c[i], beta * x[i]);
     }
     s++;
     if (trace && b < o->trace_chains && o->trace_thin > 0 && s % o->trace_thin == 0) {
@@ -324,9 +338,11 @@ int oracle_sample(int n, int m, int W, int d, const double *warm, const double *
                   const double *N, int P, int S, int b_lo, int b_hi, uint64_t seed, uint32_t file,
                   uint32_t model, int chain0, int lanes, int check_every, int max_misses, double dTol,
                   double uTol, double maxMinTol, double reprojTol, int threads, double *out,
-                  int32_t *stats, double *trace, int trace_chains, int trace_thin, int trace_len) {
+                  int32_t *stats, double *trace, int trace_chains, int trace_thin, int trace_len,
+                  const double *bank, int K) {
   if (lanes < 1 || (lanes & (lanes - 1)) || (lanes > 64 && lanes % 64)) return -1;
-  oracle_model M = {n, m, W, d, warm, lb, ub, c0, rowptr, colidx, vals, N};
+  if (bank && K <= 0) return -1;
+  oracle_model M = {n, m, W, d, warm, lb, ub, c0, rowptr, colidx, vals, N, bank, K};
   oracle_opts o = {seed, file, model, chain0, lanes, check_every, max_misses, dTol, uTol, maxMinTol,
                    reprojTol, trace_chains, trace_thin, trace_len};
   const size_t wk_len = (size_t)4 * n + (size_t)m + (size_t)d + 8;

@@ -35,7 +35,8 @@ def lib():
         h.oracle_sample.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, PD, PD, PD, PD, PI, PI, PD, PD,
                                     C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32,
                                     C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
-                                    C.c_double, C.c_int, PD, C.POINTER(C.c_int32), PD, C.c_int, C.c_int, C.c_int]
+                                    C.c_double, C.c_int, PD, C.POINTER(C.c_int32), PD, C.c_int, C.c_int, C.c_int,
+                                    PD, C.c_int]
         h.oracle_sample.restype = C.c_int
         _lib = h
     return _lib
@@ -90,7 +91,7 @@ def csr_of(S: np.ndarray):
 
 def sample(*, warm, lb, ub, rowptr, colidx, vals, N, m, P, S, seed=0, file=0, model=0, chain0=0, lanes=64,
            check_every=10, max_misses=100000, dTol=1e-14, uTol=1e-9, maxMinTol=1e-9, reprojTol=1e-9,
-           threads=1, b_lo=0, b_hi=None, trace_chains=0, trace_thin=0, c0=None):
+           threads=1, b_lo=0, b_hi=None, trace_chains=0, trace_thin=0, c0=None, bank=None):
     """Runs chains [b_lo, b_hi) of a P-chain file; returns (points n x P, stats P x 4, trace)."""
     warm = np.asfortranarray(warm, dtype=np.float64)
     n, W = warm.shape
@@ -111,12 +112,19 @@ def sample(*, warm, lb, ub, rowptr, colidx, vals, N, m, P, S, seed=0, file=0, mo
     stats = np.zeros((P, 4), dtype=np.int32)
     L = S // trace_thin if trace_thin > 0 else 0
     trace = np.zeros((max(trace_chains, 1), max(L, 1), n)) if trace_chains and L else None
+    if bank is not None:
+        bank = np.asfortranarray(bank, dtype=np.float64)
+        bflat = np.ascontiguousarray(bank.ravel(order="F"))
+        K = bank.shape[1]
+    else:
+        bflat, K = None, 0
     rc = lib().oracle_sample(n, m, W, d, _dp(wflat), _dp(lb), _dp(ub), _dp(np.ascontiguousarray(c0)),
                              rowptr.ctypes.data_as(C.POINTER(C.c_int)), colidx.ctypes.data_as(C.POINTER(C.c_int)),
                              _dp(vals), _dp(Nflat), P, S, b_lo, b_hi, seed, file, model, chain0, lanes,
                              check_every, max_misses, dTol, uTol, maxMinTol, reprojTol, threads, _dp(out),
                              stats.ctypes.data_as(C.POINTER(C.c_int32)),
-                             _dp(trace) if trace is not None else None, trace_chains, trace_thin, L)
+                             _dp(trace) if trace is not None else None, trace_chains, trace_thin, L,
+                             _dp(bflat) if bflat is not None else None, K)
     if rc != 0:
         raise ValueError(f"oracle_sample rejected its arguments (lanes={lanes})")
     return out, stats, trace
