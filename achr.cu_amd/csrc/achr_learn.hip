@@ -349,7 +349,8 @@ int achr_sampler_learn_directions(achr_sampler* S, const achr_learn_options* opt
   const int r = std::min(o.rank, df);
   long Msamp = 0;
   double loss0 = 0.0, loss1 = 0.0, sig2 = 1.0, captured = 0.0, train_ms = 0.0;
-  DevBuf<float> Q32;     // df x r column major (Q[i + j*df])
+  DevBuf<float> Q32;     // df x r column major (Q[i + j*df]), orthonormal principal axes
+  DevBuf<float> Qs32;    // the same axes scaled by sqrt(lambda_j)
   DevBuf<float> Wmix;    // K x df row major
   HIP_TRY(Wmix.ensure(static_cast<size_t>(K) * df));
   const uint64_t seed = o.seed;
@@ -457,15 +458,14 @@ int achr_sampler_learn_directions(achr_sampler* S, const achr_learn_options* opt
       std::sort(ord.begin(), ord.end(), [&](int a, int b) {
         return A[a + static_cast<size_t>(a) * q] > A[b + static_cast<size_t>(b) * q];
       });
-      // Q_r = Q V[:, top r]; target C_r = diag(top r eigenvalues)
+      // Q_r = Q V[:, top r] (principal axes), lambda_j their variances.
       std::vector<float> Qr(static_cast<size_t>(df) * r, 0.0f);
-      std::vector<float> Cr(static_cast<size_t>(r) * r, 0.0f);
+      std::vector<double> lam(r);
       captured = 0.0;
       for (int j = 0; j < r; ++j) {
         const int src = ord[j];
-        const double lam = std::max(0.0, A[src + static_cast<size_t>(src) * q]);
-        Cr[static_cast<size_t>(j) * r + j] = static_cast<float>(lam);
-        captured += lam;
+        lam[j] = std::max(0.0, A[src + static_cast<size_t>(src) * q]);
+        captured += lam[j];
         for (int i = 0; i < df; ++i) {
           double s = 0.0;
           for (int k = 0; k < q; ++k) s += G[static_cast<size_t>(i) + static_cast<size_t>(k) * df] * V[k + static_cast<size_t>(src) * q];
@@ -474,14 +474,24 @@ int achr_sampler_learn_directions(achr_sampler* S, const achr_learn_options* opt
       }
       sig2 = df > r ? std::max(tot - captured, 0.0) / (df - r) : 0.0;
       captured = tot > 0.0 ? captured / tot : 0.0;
-      // The network models the whitened law in these axes, scaled so the
-      // target has unit mean eigenvalue.
+      // The network emits whitened directions (target: identity second
+      // moment); the rounding map scales principal axis j by sqrt(lambda_j).
+      // Working in whitened axes keeps the fit relative per axis, so the
+      // 1e5 : 1 variance ratios of internal cycles do not leak into the
+      // narrow directions. Scales are normalised to unit mean variance.
       HIP_TRY(Q32.upload(Qr.data(), Qr.size(), nullptr));
       double mean_lam = 0.0;
-      for (int j = 0; j < r; ++j) mean_lam += Cr[static_cast<size_t>(j) * r + j];
+      for (int j = 0; j < r; ++j) mean_lam += lam[j];
       mean_lam /= r;
       const double scale = mean_lam > 0.0 ? 1.0 / mean_lam : 1.0;
-      for (auto& v : Cr) v = static_cast<float>(v * scale);
+      std::vector<float> Qs(Qr.size());
+      for (int j = 0; j < r; ++j) {
+        const float sj = static_cast<float>(std::sqrt(lam[j] * scale));
+        for (int i = 0; i < df; ++i) Qs[static_cast<size_t>(i) + static_cast<size_t>(j) * df] = Qr[static_cast<size_t>(i) + static_cast<size_t>(j) * df] * sj;
+      }
+      HIP_TRY(Qs32.upload(Qs.data(), Qs.size(), nullptr));
+      std::vector<float> Cr(static_cast<size_t>(r) * r, 0.0f);
+      for (int j = 0; j < r; ++j) Cr[static_cast<size_t>(j) * r + j] = 1.0f;
       sig2 *= scale;
       // Train the network.
       achr_dirnet* net = nullptr;
@@ -515,10 +525,9 @@ int achr_sampler_learn_directions(achr_sampler* S, const achr_learn_options* opt
         HIP_TRY(achr::launch_gauss(Wmix.p, static_cast<size_t>(K) * df, seed, 0xB2u, nullptr));
         // GQ = G Q (K x r)
         GEMM_TRY(gemm(K, r, df, Wmix.p, df, 1, Q32.p, 1, df, GQ.p, r, 1), achr::kPrecBF16x3, K);
-        // GQ <- V - sigma GQ, then Wmix <- sigma Wmix + GQ Q^T
-        HIP_TRY(achr::launch_axpby(GQ.p, Vb.p, -sig, 1.0f, static_cast<size_t>(K) * r, nullptr));
-        // C(k, i) with B(j, i) = Q[i + j*df]
-        GEMM_TRY(gemm(K, df, r, GQ.p, r, 1, Q32.p, df, 1, Wmix.p, df, 1, 1.0f, sig), achr::kPrecBF16x3, K);
+        // Wmix <- sigma (Wmix - GQ Q^T) + V Qs^T
+        GEMM_TRY(gemm(K, df, r, GQ.p, r, 1, Q32.p, df, 1, Wmix.p, df, 1, -sig, sig), achr::kPrecBF16x3, K);
+        GEMM_TRY(gemm(K, df, r, Vb.p, r, 1, Qs32.p, df, 1, Wmix.p, df, 1, 1.0f, 1.0f), achr::kPrecBF16x3, K);
       }
       achr_dirnet_destroy(net);
       if (rc != ACHR_OK) return rc;

@@ -303,22 +303,24 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
         u[k] = ui;
         const double du = bload(r_ub, off) - x[k];
         const double dl = x[k] - bload(r_lb, off);
-        if (du > a.dTol && dl > a.dTol) {
-          if (ui > a.uTol) {
-            double v = -dl / ui;
-            if (v > mn) mn = v;
-            v = du / ui;
-            if (v < mx) mx = v;
-          } else if (ui < -a.uTol) {
-            double v = du / ui;
-            if (v > mn) mn = v;
-            v = -dl / ui;
-            if (v < mx) mx = v;
-          }
+        // Both branches of the reference (ACHRCuda.cu:270-279) divide the same
+        // two distances by u_i; divide once and select, so a wavefront whose
+        // lanes disagree on the sign does not run both sets of divisions.
+        const double r_lo = -dl / ui;
+        const double r_hi = du / ui;
+        const bool keep = du > a.dTol && dl > a.dTol;
+        const bool pos = keep && ui > a.uTol;
+        const bool neg = keep && ui < -a.uTol;
+        const double lo = pos ? r_lo : (neg ? r_hi : -INFINITY);
+        const double hi = pos ? r_hi : (neg ? r_lo : INFINITY);
+        if (lo > mn) mn = lo;
+        if (hi < mx) mx = hi;
+        // Schedule four slots at a time: their bound loads overlap, while
+        // hoisting every slot's loads and FP64 divisions ahead would multiply
+        // the live registers by E.
+        if constexpr (E > 4) {
+          if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
-        // Keep the schedule slot by slot: hoisting every slot's bound loads
-        // and FP64 divisions ahead multiplies the live registers by E.
-        if constexpr (E > 4) __builtin_amdgcn_sched_barrier(0);
       }
       group_minmax<TPC>(mn, mx, red + kRedSlots, red + 2 * kRedSlots);
       const bool miss = !(nrm > 0.0) || !(mn > -INFINITY) || !(mx < INFINITY) ||
@@ -398,7 +400,9 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
       } else {
         c[k] = fma(alpha, c[k], beta * x[k]);
       }
-      if constexpr (E > 4) __builtin_amdgcn_sched_barrier(0);
+      if constexpr (E > 4) {
+        if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      }
     }
     ++s;
     if (a.trace != nullptr && b < a.trace_chains && s % a.trace_thin == 0) {

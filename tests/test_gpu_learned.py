@@ -44,53 +44,43 @@ def cycle_model(seed=11, cycles=10, m_rest=40, n_rest=100):
     return S, lb, ub, model, W
 
 
-def _ess_per_step(smp, P, S, seed):
-    smp.run(P, S, seed=seed)
+def _cycle_error(smp, P, steps, seed):
+    """Runs the sampler; returns (moment error of the cycle fluxes against
+    their closed form U(-1000, 1000), final points, traces)."""
+    smp.run(P, steps, seed=seed)
     tr = smp.trace()
-    live = (smp_bounds[1] - smp_bounds[0]) > 0
-    return st.bulk_ess(tr[:, :, live]), tr
-
-
-smp_bounds = None
+    a = tr[:, tr.shape[1] // 2:, :20].reshape(-1, 20)
+    err = max(np.abs(a.mean(0)).max() / 2000.0, (np.abs(a.std(0) - 2000 / np.sqrt(12)) / (2000 / np.sqrt(12))).max())
+    return err, tr
 
 
 def test_learned_bank_no_drift_same_law_and_faster_mixing():
-    global smp_bounds
+    """Chains start at the FVA warmup centre, where every cycle flux sits at
+    -983: the sampler has to travel the long cycle directions to reach the
+    uniform law. Isotropic hit-and-run (the method without the network)
+    barely moves along them; the learned rounding map does."""
     S, lb, ub, model, W = cycle_model()
-    smp_bounds = (lb, ub)
-    live = (ub - lb) > 0
-    P, steps = 256, 2000
+    P, steps = 256, 6000
     opts = achr.Options(lanes=64, trace_chains=P, trace_thin=20, check_every=100)
-    print("plain ACHR training run", flush=True)
-    plain = achr.Sampler(model, W, opts)
-    plain.run(P, steps, seed=1)
-    print("learn", flush=True)
-    rep = plain.learn_directions(rank=16, width=128, iters=300, batch=1024, lr=2e-3, seed=5)
-    assert rep["loss_end"] < rep["loss_start"]
-    print(rep, flush=True)
-    learned_ess, tr_l = _ess_per_step(plain, P, steps, 2)
-    pts_l = tr_l[:, -1, :].T
-    iso = achr.Sampler(model, W, opts)
-    iso.run(P, 10, seed=0)
-    iso.learn_directions(isotropic=True, seed=5)
-    iso_ess, tr_i = _ess_per_step(iso, P, steps, 2)
-    # 1. bank directions keep S x = 0 (after the first FP64 check removes the
-    #    warmup centre's "%f" residual)
-    assert np.abs(S @ pts_l).max() < 1e-9
-    # 2. stationary law: the polytope is the product of the cycle cube
-    #    [-1000, 1000]^10 and the rest, so each cycle flux is U(-1000, 1000):
-    #    mean 0, std 2000/sqrt(12), small KS distance to the uniform law.
-    a = tr_l[:, tr_l.shape[1] // 2:, :20].reshape(-1, 20)
-    assert np.all(np.abs(a.mean(0)) < 0.05 * 2000)
-    np.testing.assert_allclose(a.std(0), 2000 / np.sqrt(12), rtol=0.08)
-    for j in range(0, 20, 2):
-        z = np.sort((a[:, j] + 1000) / 2000)
-        assert np.max(np.abs(np.arange(1, len(z) + 1) / len(z) - z)) < 0.08
-    # 3. the trained map mixes faster than isotropic directions
-    gain = np.nanmin(learned_ess) / np.nanmin(iso_ess)
-    print(f"min-ESS learned {np.nanmin(learned_ess):.1f} isotropic {np.nanmin(iso_ess):.1f} gain {gain:.2f}",
-          flush=True)
-    assert gain > 1.5
+    errs = {}
+    for name, kw in (("learned", dict(rank=16, width=128, iters=300, batch=1024, lr=2e-3, seed=5)),
+                     ("untrained", dict(rank=16, width=128, iters=0, seed=5)),
+                     ("isotropic", dict(isotropic=True, seed=5))):
+        print(f"{name}: ACHR training run + learn", flush=True)
+        smp = achr.Sampler(model, W, opts)
+        smp.run(P, 2000, seed=1)
+        rep = smp.learn_directions(**kw)
+        if name == "learned":
+            assert rep["loss_end"] < 0.2 * rep["loss_start"]
+        err, tr = _cycle_error(smp, P, steps, 2)
+        errs[name] = err
+        print(name, rep, f"cycle moment error {err:.4f}", flush=True)
+        # bank directions keep S x = 0 (the first check removes the warmup
+        # centre's "%f" residual)
+        assert np.abs(S @ tr[:, -1, :].T).max() < 1e-9
+    assert errs["learned"] < 0.15
+    assert errs["isotropic"] > 0.5
+    assert errs["learned"] < 0.5 * errs["untrained"]
 
 
 def test_bank_mode_bitwise_against_oracle(c1):
