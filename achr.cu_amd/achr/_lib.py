@@ -75,6 +75,14 @@ SIGNATURES = {
     "achr_sampler_set_directions": (_I, [_P, _PD, _I]),
     "achr_sampler_use_directions": (_I, [_P, _I]),
     "achr_sampler_set_run_options": (_I, [_P, _I, _I, _I]),
+    "achr_batch_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
+    "achr_batch_destroy": (None, [_P]),
+    "achr_batch_set_stream": (_I, [_P, _P]),
+    "achr_batch_autotune": (_I, [_P, _PI, _I, C.c_uint64, _PD]),
+    "achr_batch_lanes": (_I, [_P, _PI]),
+    "achr_batch_set_lanes": (_I, [_P, _PI]),
+    "achr_batch_sample": (_I, [_P, _PI, _I, C.c_uint64, C.c_uint32, _I]),
+    "achr_batch_timing": (_I, [_P, _PD, _PI, _PI, _PI, _PI]),
     "achr_gemm_f32": (_I, [_I, _I, _I, C.POINTER(C.c_float), C.c_long, C.c_long, C.POINTER(C.c_float), C.c_long,
                            C.c_long, C.POINTER(C.c_float), C.c_long, C.c_long, C.c_float, C.c_float, _I]),
 }
@@ -96,6 +104,7 @@ class AchrOptions(C.Structure):
         ("trace_thin", C.c_int),
         ("proj_prec", C.c_int),
         ("fp64_every", C.c_int),
+        ("model_id", C.c_int),
     ]
 
 

@@ -134,11 +134,12 @@ class Options:
     trace_thin: int = 0
     proj_prec: int = 0
     fp64_every: int = 10
+    model_id: int = 0
 
     def to_c(self) -> AchrOptions:
         return AchrOptions(self.dTol, self.uTol, self.maxMinTol, self.reprojTol, self.check_every,
                            self.max_misses, self.lanes, self.reproject_mode, self.trace_chains, self.trace_thin,
-                           self.proj_prec, self.fp64_every)
+                           self.proj_prec, self.fp64_every, self.model_id)
 
 
 PROJ_FP64, PROJ_RES_BF16X3, PROJ_RES_F32, PROJ_NULL_F32, PROJ_NULL_BF16X3 = range(5)
@@ -243,6 +244,62 @@ class Sampler:
 
     def use_directions(self, on: bool) -> None:
         check(lib().achr_sampler_use_directions(self._h, 1 if on else 0), "achr_sampler_use_directions")
+
+
+class Batch:
+    """Several samplers run as one job (achr_batch_*): one persistent launch
+    per schedule bucket and segment, each segment replayed from a HIP graph."""
+
+    TPC = (16, 32, 64, 128, 256, 512, 1024)
+
+    def __init__(self, samplers):
+        self.samplers = list(samplers)
+        arr = (C.c_void_p * len(self.samplers))(*[s._h.value for s in self.samplers])
+        h = C.c_void_p()
+        check(lib().achr_batch_create(arr, len(self.samplers), C.byref(h)), "achr_batch_create")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().achr_batch_destroy(h)
+            self._h = None
+
+    def _ints(self, v):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.int32), (len(self.samplers),)))
+        return a, a.ctypes.data_as(C.POINTER(C.c_int))
+
+    def autotune(self, P, probe_steps: int = 20, seed: int = 0) -> np.ndarray:
+        """Times every candidate threads-per-chain per size class; returns the
+        probe times (models x 7, -1 where not a candidate)."""
+        _, pp = self._ints(P)
+        ms = np.full((len(self.samplers), len(self.TPC)), -1.0)
+        check(lib().achr_batch_autotune(self._h, pp, probe_steps, C.c_uint64(seed), _dp(ms)), "achr_batch_autotune")
+        return ms
+
+    def lanes(self) -> np.ndarray:
+        out = np.zeros(len(self.samplers), dtype=np.int32)
+        check(lib().achr_batch_lanes(self._h, out.ctypes.data_as(C.POINTER(C.c_int))), "achr_batch_lanes")
+        return out
+
+    def set_lanes(self, lanes) -> None:
+        _, pp = self._ints(lanes)
+        check(lib().achr_batch_set_lanes(self._h, pp), "achr_batch_set_lanes")
+
+    def run(self, P, S: int, seed: int = 0, file: int = 0, graph: bool = True) -> None:
+        a, pp = self._ints(P)
+        check(lib().achr_batch_sample(self._h, pp, S, C.c_uint64(seed), C.c_uint32(file), 1 if graph else 0),
+              "achr_batch_sample")
+        for smp, p in zip(self.samplers, a.tolist()):
+            smp.P, smp.S = p, S
+
+    def timing(self) -> dict:
+        ms = C.c_double()
+        seg, bk, gr, nodes = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().achr_batch_timing(self._h, C.byref(ms), C.byref(seg), C.byref(bk), C.byref(gr), C.byref(nodes)),
+              "achr_batch_timing")
+        return {"ms_total": ms.value, "segments": seg.value, "buckets": bk.value, "graphs": gr.value,
+                "graph_nodes": nodes.value}
 
 
 def philox4x32_10(ctr, key) -> tuple:

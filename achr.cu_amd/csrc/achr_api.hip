@@ -71,6 +71,7 @@ void achr_options_default(achr_options* o) {
   o->trace_thin = 0;
   o->proj_prec = ACHR_PROJ_FP64;
   o->fp64_every = 10;
+  o->model_id = 0;
 }
 
 // ------------------------------------------------------------------ model
@@ -533,6 +534,7 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
   if (o.trace_chains > 0 && o.trace_thin <= 0) return fail(ACHR_E_ARG, "trace_thin must be > 0");
   if (o.proj_prec < ACHR_PROJ_FP64 || o.proj_prec > ACHR_PROJ_NULL_BF16X3 || o.fp64_every < 1)
     return fail(ACHR_E_ARG, "bad proj_prec %d / fp64_every %d", o.proj_prec, o.fp64_every);
+  if (o.model_id < 0 || o.model_id >= (1 << 27)) return fail(ACHR_E_ARG, "model_id %d outside 0..2^27-1", o.model_id);
   if (o.proj_prec != ACHR_PROJ_FP64 && o.reproject_mode == 1)
     return fail(ACHR_E_ARG, "reduced-precision projection runs in batched rounds (reproject_mode 2)");
   auto* S = new (std::nothrow) achr_sampler();
@@ -702,18 +704,36 @@ static int event_at(achr_sampler* S, size_t i, hipEvent_t* ev) {
   return ACHR_OK;
 }
 
-int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0) {
-  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
-  if (P <= 0 || Ssteps < 0 || chain0 < 0) return fail(ACHR_E_ARG, "bad P=%d S=%d chain0=%d", P, Ssteps, chain0);
-  const int n = S->n;
-  const size_t Pn = static_cast<size_t>(P) * n;
-  if (S->sched.tpc < 64) {
+}  // extern "C"
+
+namespace achr_host {
+
+int run_limits(const achr_sampler* S, const achr::Schedule& sc, int P, int Ssteps) {
+  if (S->sched.e < 0 || sc.e < 0) return fail(ACHR_E_ARG, "no register schedule for n=%d", S->n);
+  if (sc.tpc < 64) {
+    // Groups smaller than a wavefront address per-chain rows and the warmup
+    // set with 32-bit byte offsets from the array base.
     const double lim = 2147483648.0;
     const double trace_rows = static_cast<double>(std::min(S->opt.trace_chains, P)) *
                               (S->opt.trace_thin > 0 ? Ssteps / S->opt.trace_thin : 0);
-    if (8.0 * P * n >= lim || 8.0 * P * std::max(S->d, 1) >= lim || 8.0 * trace_rows * n >= lim)
-      return fail(ACHR_E_ARG, "P=%d too large for a %d-lane schedule (32-bit row offsets)", P, S->sched.tpc);
+    const int W = S->use_dirs ? S->K : S->W;
+    if (8.0 * P * S->n >= lim || 8.0 * P * std::max(S->d, 1) >= lim || 8.0 * trace_rows * S->n >= lim ||
+        8.0 * S->n * W >= lim)
+      return fail(ACHR_E_ARG, "P=%d too large for a %d-lane schedule (32-bit row offsets)", P, sc.tpc);
   }
+  return ACHR_OK;
+}
+
+int segments_for(const achr_sampler* S, int Ssteps) {
+  const int checks = (Ssteps + S->opt.check_every - 1) / S->opt.check_every;
+  return (S->mode == 2) ? checks + 1 : 1;
+}
+
+int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0, hipStream_t st,
+                achr::StepArgs* args) {
+  if (P <= 0 || Ssteps < 0 || chain0 < 0) return fail(ACHR_E_ARG, "bad P=%d S=%d chain0=%d", P, Ssteps, chain0);
+  const int n = S->n;
+  const size_t Pn = static_cast<size_t>(P) * n;
   const int tr_chains = std::min(S->opt.trace_chains, P);
   const int tr_len = S->opt.trace_thin > 0 ? Ssteps / S->opt.trace_thin : 0;
   HIP_TRY(S->X.ensure(Pn));
@@ -736,12 +756,14 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   S->P = P;
   S->S = Ssteps;
   S->trace_len = (tr_chains > 0) ? tr_len : 0;
-  hipStream_t st = S->stream;
   HIP_TRY(hipMemsetAsync(S->st.p, 0, sizeof(int4) * P, st));
   HIP_TRY(hipMemsetAsync(S->stats.p, 0, sizeof(int4) * P, st));
   HIP_TRY(hipMemsetAsync(S->flag.p, 0, sizeof(int) * P, st));
+  S->ev_step.clear();
+  S->ev_proj.clear();
 
-  achr::StepArgs a{};
+  achr::StepArgs& a = *args;
+  a = achr::StepArgs{};
   a.n = n;
   a.m = S->m;
   a.W = S->use_dirs ? S->K : S->W;
@@ -760,7 +782,7 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   a.S = Ssteps;
   a.seed = seed;
   a.file = file;
-  a.model = 0;
+  a.model = static_cast<uint32_t>(S->opt.model_id);
   a.chain0 = chain0;
   a.dTol = S->opt.dTol;
   a.uTol = S->opt.uTol;
@@ -782,100 +804,129 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   S->trace_chains_run = S->trace_len > 0 ? tr_chains : 0;
   a.trace_thin = S->opt.trace_thin;
   a.trace_len = S->trace_len;
+  return ACHR_OK;
+}
 
-  S->ev_step.clear();
-  S->ev_proj.clear();
+int projection_kind(const achr_sampler* S, int seg) {
+  if (S->mode != 2 || S->d <= 0) return kProjNone;
+  const int prec = S->opt.proj_prec;
+  if (prec == ACHR_PROJ_FP64 || (seg + 1) % S->opt.fp64_every == 0) return kProjFP64;
+  return kProjLow;
+}
+
+int enqueue_projection(achr_sampler* S, int kind, hipStream_t st) {
+  if (kind == kProjNone) return ACHR_OK;
+  const int n = S->n;
+  const int P = S->P;
+  const int prec = S->opt.proj_prec;
+  const bool res_form = prec == ACHR_PROJ_RES_BF16X3 || prec == ACHR_PROJ_RES_F32;
+  const bool fp64_round = kind == kProjFP64;
+  hipError_t le = hipSuccess;
+  le = achr::launch_compact(S->flag.p, P, S->idx.p, S->count.p, st);
+  if (le == hipSuccess && fp64_round) {
+    le = achr::launch_project(S->Ncm.p, n, S->d, S->X.p, S->T.p, S->idx.p, S->count.p, P, st);
+  } else if (le == hipSuccess && res_form) {
+    // X(q, :) -= sum_k R(q, k) S+(:, k): A = residual rows, B(k, i) = S+[i + k*n]
+    achr::GemmArgs g{};
+    g.M = P;
+    g.N = n;
+    g.K = S->m;
+    g.Mdev = S->count.p;
+    g.A = S->R32.p;
+    g.sa_m = S->m;
+    g.sa_k = 1;
+    g.idxA = S->idx.p;
+    g.B = S->Sp32.p;
+    g.sb_k = n;
+    g.sb_n = 1;
+    g.C = S->X.p;
+    g.sc_m = n;
+    g.sc_n = 1;
+    g.idxC = S->idx.p;
+    g.alpha = 1.0f;
+    g.epi = achr::kEpiSubF64;
+    le = achr::launch_gemm(g, prec == ACHR_PROJ_RES_F32 ? achr::kPrecF32 : achr::kPrecBF16x3, P, st);
+  } else if (le == hipSuccess) {
+    const int gp = prec == ACHR_PROJ_NULL_F32 ? achr::kPrecF32 : achr::kPrecBF16x3;
+    // T(q, j) = sum_i X(q, i) N(i, j)
+    achr::GemmArgs g{};
+    g.M = P;
+    g.N = S->d;
+    g.K = n;
+    g.Mdev = S->count.p;
+    g.A = S->X.p;
+    g.a_f64 = 1;
+    g.sa_m = n;
+    g.sa_k = 1;
+    g.idxA = S->idx.p;
+    g.B = S->N32.p;
+    g.sb_k = 1;
+    g.sb_n = n;
+    g.C = S->T32.p;
+    g.sc_m = S->d;
+    g.sc_n = 1;
+    g.alpha = 1.0f;
+    g.epi = achr::kEpiStoreF32;
+    le = achr::launch_gemm(g, gp, P, st);
+    // X(q, i) = sum_j T(q, j) N(i, j)
+    achr::GemmArgs h{};
+    h.M = P;
+    h.N = n;
+    h.K = S->d;
+    h.Mdev = S->count.p;
+    h.A = S->T32.p;
+    h.sa_m = S->d;
+    h.sa_k = 1;
+    h.B = S->N32.p;
+    h.sb_k = n;
+    h.sb_n = 1;
+    h.C = S->X.p;
+    h.sc_m = n;
+    h.sc_n = 1;
+    h.idxC = S->idx.p;
+    h.alpha = 1.0f;
+    h.epi = achr::kEpiStoreF64;
+    if (le == hipSuccess) le = achr::launch_gemm(h, gp, P, st);
+  }
+  if (le != hipSuccess) return fail(ACHR_E_DEVICE, "projection launch: %s", hipGetErrorString(le));
+  return ACHR_OK;
+}
+
+}  // namespace achr_host
+
+extern "C" {
+
+int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  int rc = achr_host::run_limits(S, S->sched, P, Ssteps);
+  if (rc != ACHR_OK) return rc;
+  hipStream_t st = S->stream;
+  achr::StepArgs a{};
+  rc = achr_host::prepare_run(S, P, Ssteps, seed, file, chain0, st, &a);
+  if (rc != ACHR_OK) return rc;
   size_t evn = 0;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventRecord(S->ev_begin, st));
-  const int checks = (Ssteps + S->opt.check_every - 1) / S->opt.check_every;
-  const int segments = (S->mode == 2) ? checks + 1 : 1;
+  const int segments = achr_host::segments_for(S, Ssteps);
   for (int seg = 0; seg < segments; ++seg) {
-    int rc = event_at(S, evn, &e0);
+    rc = event_at(S, evn, &e0);
     if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
     if (rc != ACHR_OK) return rc;
     HIP_TRY(hipEventRecord(e0, st));
-    hipError_t le = achr::launch_step(a, S->sched, st);
+    const hipError_t le = achr::launch_step(a, S->sched, st);
     if (le != hipSuccess) return fail(ACHR_E_DEVICE, "step kernel launch: %s", hipGetErrorString(le));
     HIP_TRY(hipEventRecord(e1, st));
     S->ev_step.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
     evn += 2;
-    if (S->mode == 2 && seg + 1 < segments && S->d > 0) {
+    const int kind = seg + 1 < segments ? achr_host::projection_kind(S, seg) : achr_host::kProjNone;
+    if (kind != achr_host::kProjNone) {
       rc = event_at(S, evn, &e0);
       if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
       if (rc != ACHR_OK) return rc;
       HIP_TRY(hipEventRecord(e0, st));
-      le = achr::launch_compact(S->flag.p, P, S->idx.p, S->count.p, st);
-      const bool fp64_round = prec == ACHR_PROJ_FP64 || (seg + 1) % S->opt.fp64_every == 0;
-      if (le == hipSuccess && fp64_round) {
-        le = achr::launch_project(S->Ncm.p, n, S->d, S->X.p, S->T.p, S->idx.p, S->count.p, P, st);
-        ++S->rounds_fp64;
-      } else if (le == hipSuccess && res_form) {
-        // X(q, :) -= sum_k R(q, k) S+(:, k): A = residual rows, B(k, i) = S+[i + k*n]
-        achr::GemmArgs g{};
-        g.M = P;
-        g.N = n;
-        g.K = S->m;
-        g.Mdev = S->count.p;
-        g.A = S->R32.p;
-        g.sa_m = S->m;
-        g.sa_k = 1;
-        g.idxA = S->idx.p;
-        g.B = S->Sp32.p;
-        g.sb_k = n;
-        g.sb_n = 1;
-        g.C = S->X.p;
-        g.sc_m = n;
-        g.sc_n = 1;
-        g.idxC = S->idx.p;
-        g.alpha = 1.0f;
-        g.epi = achr::kEpiSubF64;
-        le = achr::launch_gemm(g, prec == ACHR_PROJ_RES_F32 ? achr::kPrecF32 : achr::kPrecBF16x3, P, st);
-        ++S->rounds_low;
-      } else if (le == hipSuccess) {
-        const int gp = prec == ACHR_PROJ_NULL_F32 ? achr::kPrecF32 : achr::kPrecBF16x3;
-        // T(q, j) = sum_i X(q, i) N(i, j)
-        achr::GemmArgs g{};
-        g.M = P;
-        g.N = S->d;
-        g.K = n;
-        g.Mdev = S->count.p;
-        g.A = S->X.p;
-        g.a_f64 = 1;
-        g.sa_m = n;
-        g.sa_k = 1;
-        g.idxA = S->idx.p;
-        g.B = S->N32.p;
-        g.sb_k = 1;
-        g.sb_n = n;
-        g.C = S->T32.p;
-        g.sc_m = S->d;
-        g.sc_n = 1;
-        g.alpha = 1.0f;
-        g.epi = achr::kEpiStoreF32;
-        le = achr::launch_gemm(g, gp, P, st);
-        // X(q, i) = sum_j T(q, j) N(i, j)
-        achr::GemmArgs h{};
-        h.M = P;
-        h.N = n;
-        h.K = S->d;
-        h.Mdev = S->count.p;
-        h.A = S->T32.p;
-        h.sa_m = S->d;
-        h.sa_k = 1;
-        h.B = S->N32.p;
-        h.sb_k = n;
-        h.sb_n = 1;
-        h.C = S->X.p;
-        h.sc_m = n;
-        h.sc_n = 1;
-        h.idxC = S->idx.p;
-        h.alpha = 1.0f;
-        h.epi = achr::kEpiStoreF64;
-        if (le == hipSuccess) le = achr::launch_gemm(h, gp, P, st);
-        ++S->rounds_low;
-      }
-      if (le != hipSuccess) return fail(ACHR_E_DEVICE, "projection launch: %s", hipGetErrorString(le));
+      rc = achr_host::enqueue_projection(S, kind, st);
+      if (rc != ACHR_OK) return rc;
+      if (kind == achr_host::kProjFP64) ++S->rounds_fp64; else ++S->rounds_low;
       HIP_TRY(hipEventRecord(e1, st));
       S->ev_proj.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
       evn += 2;
@@ -889,7 +940,7 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
 int achr_sample_points(achr_sampler* S, double* out) {
   if (!S || !out) return fail(ACHR_E_ARG, "NULL argument");
   if (!S->have_run) return fail(ACHR_E_STATE, "no run");
-  HIP_TRY(hipStreamSynchronize(S->stream));
+  HIP_TRY(hipEventSynchronize(S->ev_end));
   HIP_TRY(hipMemcpy(out, S->out.p, sizeof(double) * static_cast<size_t>(S->P) * S->n, hipMemcpyDeviceToHost));
   return ACHR_OK;
 }
@@ -904,7 +955,7 @@ int achr_sample_points_device(achr_sampler* S, void** dev_ptr) {
 int achr_sample_stats(achr_sampler* S, int32_t* stats) {
   if (!S || !stats) return fail(ACHR_E_ARG, "NULL argument");
   if (!S->have_run) return fail(ACHR_E_STATE, "no run");
-  HIP_TRY(hipStreamSynchronize(S->stream));
+  HIP_TRY(hipEventSynchronize(S->ev_end));
   HIP_TRY(hipMemcpy(stats, S->stats.p, sizeof(int4) * S->P, hipMemcpyDeviceToHost));
   return ACHR_OK;
 }
@@ -913,7 +964,7 @@ int achr_sample_trace(achr_sampler* S, double* trace) {
   if (!S || !trace) return fail(ACHR_E_ARG, "NULL argument");
   if (!S->have_run) return fail(ACHR_E_STATE, "no run");
   if (S->trace_len == 0) return fail(ACHR_E_STATE, "no trace recorded (trace_chains / trace_thin)");
-  HIP_TRY(hipStreamSynchronize(S->stream));
+  HIP_TRY(hipEventSynchronize(S->ev_end));
   const size_t cnt = static_cast<size_t>(S->trace_chains_run) * S->trace_len * S->n;
   HIP_TRY(hipMemcpy(trace, S->trace.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
   return ACHR_OK;

@@ -22,6 +22,20 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), count(o.count) {
+    o.p = nullptr;
+    o.count = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      count = o.count;
+      o.p = nullptr;
+      o.count = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -93,3 +107,22 @@ struct achr_sampler {
   std::vector<std::pair<int, int>> ev_step, ev_proj;  // indices into ev_pool
   bool have_run = false;
 };
+
+namespace achr_host {
+
+// Run set-up and projection rounds shared by single-model runs and batches.
+enum { kProjNone = 0, kProjFP64 = 1, kProjLow = 2 };
+// Rejects run sizes a schedule cannot address.
+int run_limits(const achr_sampler* S, const achr::Schedule& sc, int P, int Ssteps);
+// Kernel launches (segments) of a run: 1 in mode 1, one per check + 1 in mode 2.
+int segments_for(const achr_sampler* S, int Ssteps);
+// Sizes the run buffers for P chains, queues the reset of the chain state on
+// st and fills the kernel arguments.
+int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0, hipStream_t st,
+                achr::StepArgs* args);
+// Projection round after segment seg (kProjNone outside mode 2).
+int projection_kind(const achr_sampler* S, int seg);
+// Queues one projection round over the chains flagged by the last segment.
+int enqueue_projection(achr_sampler* S, int kind, hipStream_t st);
+
+}  // namespace achr_host

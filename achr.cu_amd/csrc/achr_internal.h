@@ -21,7 +21,7 @@ struct StepArgs {
   // run
   int P, S;
   uint64_t seed;
-  uint32_t file, model;
+  uint32_t file, model;  // Philox counter words 2 and 3 (model << 4 | tag)
   int chain0;
   double dTol, uTol, maxMinTol, reprojTol;
   int check_every, max_misses;
@@ -52,8 +52,15 @@ struct Schedule {
 
 bool schedule_supported(int tpc, int e);
 int elems_for(int n, int tpc);  // smallest instantiated E >= ceil(n / tpc), or -1
-size_t step_lds_bytes(const StepArgs& a, const Schedule& s);
+size_t step_lds_bytes(bool dir_mode, const Schedule& s);
 hipError_t launch_step(const StepArgs& a, const Schedule& s, hipStream_t stream);
+// Multi-model persistent form. prepare_step_batch opts the kernel of this
+// schedule into its LDS size and returns how many of its workgroups stay
+// resident on the device; launch_step_batch runs `grid` workgroups over
+// `nitems` items {model index into tab, first chain} of s.cpb chains each.
+hipError_t prepare_step_batch(const Schedule& s, bool dir_mode, int* resident);
+hipError_t launch_step_batch(const StepArgs* tab, const int2* items, int nitems, int grid, const Schedule& s,
+                             bool dir_mode, hipStream_t stream);
 
 // Batched reprojection: X[idx[q]] <- N (N^T X[idx[q]]) for q < *count.
 hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStream_t stream);

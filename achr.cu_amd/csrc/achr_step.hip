@@ -32,6 +32,13 @@ namespace achr {
 namespace {
 
 constexpr int kRedSlots = 16;  // wavefront partials per reduction (<= 1024 threads)
+// LDS slot arrays of the block-per-chain reductions: the norm, two pairs for
+// min/max and the residual maximum. Consecutive reductions must use disjoint
+// arrays (a wavefront may write the next partial while a slower one still
+// reads the last): with a centre the norm separates two min/max reductions;
+// without one (direction bank) the min/max pair alternates with the attempt
+// parity.
+constexpr int kRedSum = 0, kRedMinMax0 = 1, kRedMax = 3, kRedMinMax1 = 4, kRedArrays = 6;
 
 // Buffer-resource loads: 32-bit byte offsets (one VGPR per slot, shared by
 // every per-reaction array) and hardware range checking — an offset at or
@@ -167,14 +174,14 @@ struct Row {
   __device__ __forceinline__ void st(int i, double v) const { bstore(r, off(i), v); }
 };
 
+// One segment of chain b of the model `a` (b is group-uniform; with TPC > 64
+// the block is one chain, so every early return is block-uniform).
 template <int TPC, int E, int CPB, bool DIR>
-__global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void run_chain(const StepArgs& a, const int b, double* lds) {
   double* red = lds;  // 4 reduction slot arrays; only block-per-chain schedules use them
   const int g = threadIdx.x / TPC;
   const int lane = threadIdx.x % TPC;
-  const int b = blockIdx.x * CPB + g;
-  if (b >= a.P) return;  // group-uniform; with TPC > 64 the block is one chain
+  if (b >= a.P) return;
   if (a.mode == 2 && lane == 0) a.flag[b] = 0;
   int4 st = a.st[b];
   if (st.z == kDone) return;
@@ -194,7 +201,7 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
   constexpr bool kCL = !DIR && E > 8;
   double x[E];
   double c[(DIR || kCL) ? 1 : E];
-  double* cl = lds + 4 * kRedSlots + static_cast<size_t>(g) * (TPC * E);  // this chain's slice
+  double* cl = lds + kRedArrays * kRedSlots + static_cast<size_t>(g) * (TPC * E);  // this chain's slice
   if (st.z == kFresh) {
     const rsrc_t r_c0 = make_rsrc(a.c0, 8LL * n);
 #pragma unroll
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
       // Bank directions are unit vectors: no norm, no division.
       double nrm = 1.0;
       if constexpr (!DIR) {
-        ss = group_sum<TPC>(ss, red);
+        ss = group_sum<TPC>(ss, red + kRedSum * kRedSlots);
         nrm = sqrt(ss);
       }
       double mn = -INFINITY, mx = INFINITY;
@@ -322,7 +329,8 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
           if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
       }
-      group_minmax<TPC>(mn, mx, red + kRedSlots, red + 2 * kRedSlots);
+      double* rmm = red + ((DIR && (attempt & 1u)) ? kRedMinMax1 : kRedMinMax0) * kRedSlots;
+      group_minmax<TPC>(mn, mx, rmm, rmm + kRedSlots);
       const bool miss = !(nrm > 0.0) || !(mn > -INFINITY) || !(mx < INFINITY) ||
                         (fabs(mn) < a.maxMinTol && fabs(mx) < a.maxMinTol) || (mn > mx);
       if (miss) {
@@ -354,7 +362,7 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
           if (av > rho) rho = av;
           if (rrow) rrow[j] = static_cast<float>(acc);
         }
-        rho = group_max<TPC>(rho, red + 3 * kRedSlots);
+        rho = group_max<TPC>(rho, red + kRedMax * kRedSlots);
         const bool reproj = rho > a.reprojTol && d > 0;
         if (reproj) ++sts.y;
         if (a.mode == 2) {
@@ -434,34 +442,77 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
   }
 }
 
-template <int TPC, int E>
-hipError_t launch_tpc_e(const StepArgs& a, hipStream_t stream, size_t lds) {
-  constexpr int CPB = TPC <= 64 ? 256 / TPC : 1;
-  const unsigned grid = static_cast<unsigned>((a.P + CPB - 1) / CPB);
-  if (lds > 65536) {
-    // Dynamic LDS beyond 64 KiB must be opted into per kernel.
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&step_kernel<TPC, E, CPB, false>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    if (e != hipSuccess) return e;
-  }
-  if (a.dir_mode)
-    hipLaunchKernelGGL((step_kernel<TPC, E, CPB, true>), dim3(grid), dim3(TPC * CPB), lds, stream, a);
-  else
-    hipLaunchKernelGGL((step_kernel<TPC, E, CPB, false>), dim3(grid), dim3(TPC * CPB), lds, stream, a);
-  return hipGetLastError();
+template <int TPC, int E, int CPB, bool DIR>
+__global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
+  extern __shared__ double lds[];
+  run_chain<TPC, E, CPB, DIR>(a, static_cast<int>(blockIdx.x) * CPB + static_cast<int>(threadIdx.x) / TPC, lds);
 }
 
-template <int TPC>
-hipError_t launch_tpc(const StepArgs& a, int e, hipStream_t stream, size_t lds) {
-  switch (e) {
-    case 1: return launch_tpc_e<TPC, 1>(a, stream, lds);
-    case 2: return launch_tpc_e<TPC, 2>(a, stream, lds);
-    case 4: return launch_tpc_e<TPC, 4>(a, stream, lds);
-    case 8: return launch_tpc_e<TPC, 8>(a, stream, lds);
-    case 12: return launch_tpc_e<TPC, 12>(a, stream, lds);
-    case 16: return launch_tpc_e<TPC, 16>(a, stream, lds);
-    default: return hipErrorInvalidValue;
+// Several models in one persistent launch: the grid is sized to what stays
+// resident, and workgroup w takes items w, w + grid, w + 2*grid, ... of the
+// list, each {model, first chain} naming CPB consecutive chains of one model
+// (the host pads every model to whole items). The item index is uniform over
+// the workgroup, so the model's arguments load into scalar registers. A
+// block-per-chain workgroup passes a barrier between items, so the next
+// chain's first reduction cannot overwrite partials of the last one that a
+// slower wavefront still reads.
+template <int TPC, int E, int CPB, bool DIR>
+__global__ __launch_bounds__(TPC* CPB) void step_batch_kernel(const StepArgs* __restrict__ tab,
+                                                              const int2* __restrict__ items, int nitems) {
+  extern __shared__ double lds[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int2 w = items[it];
+    const StepArgs a = tab[w.x];
+    run_chain<TPC, E, CPB, DIR>(a, w.y + static_cast<int>(threadIdx.x) / TPC, lds);
+    if constexpr (TPC > 64) __syncthreads();
   }
+}
+
+template <int TPC, int E, bool DIR, bool BATCH>
+const void* kernel_for() {
+  constexpr int CPB = TPC <= 64 ? 256 / TPC : 1;
+  if constexpr (BATCH)
+    return reinterpret_cast<const void*>(&step_batch_kernel<TPC, E, CPB, DIR>);
+  else
+    return reinterpret_cast<const void*>(&step_kernel<TPC, E, CPB, DIR>);
+}
+
+template <int TPC, bool DIR, bool BATCH>
+const void* kernel_for_e(int e) {
+  switch (e) {
+    case 1: return kernel_for<TPC, 1, DIR, BATCH>();
+    case 2: return kernel_for<TPC, 2, DIR, BATCH>();
+    case 4: return kernel_for<TPC, 4, DIR, BATCH>();
+    case 8: return kernel_for<TPC, 8, DIR, BATCH>();
+    case 12: return kernel_for<TPC, 12, DIR, BATCH>();
+    case 16: return kernel_for<TPC, 16, DIR, BATCH>();
+    default: return nullptr;
+  }
+}
+
+template <bool DIR, bool BATCH>
+const void* kernel_for_tpc(int tpc, int e) {
+  switch (tpc) {
+    case 16: return kernel_for_e<16, DIR, BATCH>(e);
+    case 32: return kernel_for_e<32, DIR, BATCH>(e);
+    case 64: return kernel_for_e<64, DIR, BATCH>(e);
+    case 128: return kernel_for_e<128, DIR, BATCH>(e);
+    case 256: return kernel_for_e<256, DIR, BATCH>(e);
+    case 512: return kernel_for_e<512, DIR, BATCH>(e);
+    case 1024: return kernel_for_e<1024, DIR, BATCH>(e);
+    default: return nullptr;
+  }
+}
+
+const void* step_function(const Schedule& s, bool dir, bool batch) {
+  if (dir) return batch ? kernel_for_tpc<true, true>(s.tpc, s.e) : kernel_for_tpc<true, false>(s.tpc, s.e);
+  return batch ? kernel_for_tpc<false, true>(s.tpc, s.e) : kernel_for_tpc<false, false>(s.tpc, s.e);
+}
+
+// Dynamic LDS beyond 64 KiB must be opted into per kernel.
+hipError_t allow_lds(const void* fn, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
 }
 
 constexpr int kElems[] = {1, 2, 4, 8, 12, 16};
@@ -482,25 +533,47 @@ int elems_for(int n, int tpc) {
   return -1;
 }
 
-size_t step_lds_bytes(const StepArgs& a, const Schedule& s) {
-  size_t bytes = 4 * kRedSlots * sizeof(double);
-  if (s.e > 8 && !a.dir_mode) bytes += static_cast<size_t>(s.cpb) * s.tpc * s.e * sizeof(double);  // centres in LDS
+size_t step_lds_bytes(bool dir, const Schedule& s) {
+  size_t bytes = kRedArrays * kRedSlots * sizeof(double);
+  if (s.e > 8 && !dir) bytes += static_cast<size_t>(s.cpb) * s.tpc * s.e * sizeof(double);  // centres in LDS
   return bytes;
 }
 
 hipError_t launch_step(const StepArgs& a, const Schedule& s, hipStream_t stream) {
   if (!schedule_supported(s.tpc, s.e)) return hipErrorInvalidValue;
-  const size_t lds = step_lds_bytes(a, s);
-  switch (s.tpc) {
-    case 16: return launch_tpc<16>(a, s.e, stream, lds);
-    case 32: return launch_tpc<32>(a, s.e, stream, lds);
-    case 64: return launch_tpc<64>(a, s.e, stream, lds);
-    case 128: return launch_tpc<128>(a, s.e, stream, lds);
-    case 256: return launch_tpc<256>(a, s.e, stream, lds);
-    case 512: return launch_tpc<512>(a, s.e, stream, lds);
-    case 1024: return launch_tpc<1024>(a, s.e, stream, lds);
-    default: return hipErrorInvalidValue;
-  }
+  const bool dir = a.dir_mode != 0;
+  const void* fn = step_function(s, dir, false);
+  const size_t lds = step_lds_bytes(dir, s);
+  hipError_t e = allow_lds(fn, lds);
+  if (e != hipSuccess) return e;
+  const unsigned grid = static_cast<unsigned>((a.P + s.cpb - 1) / s.cpb);
+  StepArgs arg = a;
+  void* args[] = {&arg};
+  return hipLaunchKernel(fn, dim3(grid), dim3(s.tpc * s.cpb), args, lds, stream);
+}
+
+hipError_t prepare_step_batch(const Schedule& s, bool dir, int* resident) {
+  if (!schedule_supported(s.tpc, s.e)) return hipErrorInvalidValue;
+  const void* fn = step_function(s, dir, true);
+  const size_t lds = step_lds_bytes(dir, s);
+  hipError_t e = allow_lds(fn, lds);
+  if (e != hipSuccess) return e;
+  int per_cu = 0, dev = 0, cus = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, s.tpc * s.cpb, lds);
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  *resident = (per_cu > 0 ? per_cu : 1) * cus;
+  return hipSuccess;
+}
+
+hipError_t launch_step_batch(const StepArgs* tab, const int2* items, int nitems, int grid, const Schedule& s,
+                             bool dir, hipStream_t stream) {
+  if (!schedule_supported(s.tpc, s.e) || grid <= 0) return hipErrorInvalidValue;
+  const void* fn = step_function(s, dir, true);
+  const size_t lds = step_lds_bytes(dir, s);
+  void* args[] = {&tab, &items, &nitems};
+  return hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(s.tpc * s.cpb), args, lds, stream);
 }
 
 }  // namespace achr

@@ -116,6 +116,8 @@ typedef struct achr_options {
   int proj_prec;
   int fp64_every;       /* with a reduced-precision proj_prec: every fp64_every-th
                            round uses ACHR_PROJ_FP64 (10 = every 100 steps) */
+  int model_id;         /* Philox counter word 3 is model_id << 4 | tag: models
+                           sampled together take distinct ids (0 .. 2^27 - 1) */
 } achr_options;
 
 #define ACHR_PROJ_FP64 0
@@ -152,7 +154,7 @@ int achr_sampler_center(const achr_sampler *sampler, double *c0);
 /* Runs one output file: P chains, each started at the warmup centre, each
  * making S accepted ACHR steps (stepPointProgress, ACHRCuda.cu:361-385 and
  * the per-file loop :736-743). Uniforms come from Philox4x32-10 keyed by
- * `seed` with counter {index, chain0 + chain, file, 0}. Results stay in
+ * `seed` with counter {index, chain0 + chain, file, model_id << 4 | tag}. Results stay in
  * device memory until fetched; the call returns once the work is queued.
  */
 int achr_sample(achr_sampler *sampler, int P, int S, uint64_t seed, uint32_t file, int chain0);
@@ -182,6 +184,52 @@ int achr_sample_timing(achr_sampler *sampler, double *ms_total, double *ms_step,
 /* Schedule the autotuner picked for this sampler: threads per chain,
  * reprojection mode (1 in-kernel, 2 batched). */
 int achr_sampler_schedule(const achr_sampler *sampler, int *lanes, int *reproject_mode);
+
+/* ------------------------------------------------------------- batches */
+
+/* Several samplers (one per model, any mix of reaction counts) run as one
+ * job. Chains of the models that share a register schedule (threads per
+ * chain, slots per thread) run in one persistent launch per segment; a
+ * segment of the whole batch (every such launch, then every model's
+ * projection round) is captured once in a HIP graph and replayed, so a run
+ * costs ceil(S / check_every) + 1 graph launches however many models it
+ * holds. Every model keeps its own Philox stream (its options' model_id), so
+ * its samples equal those of achr_sample on that sampler alone with the same
+ * lanes, seed and file. Results are read per sampler (achr_sample_points,
+ * _stats, _trace). No counterpart in the reference, which runs one model per
+ * process with one child launch per step (ACHRCuda.cu:308-355). */
+typedef struct achr_batch achr_batch;
+
+/* The samplers stay owned by the caller and must outlive the batch; the
+ * batch starts with each sampler's own schedule. */
+int achr_batch_create(achr_sampler *const *samplers, int count, achr_batch **out);
+void achr_batch_destroy(achr_batch *batch);
+
+/* Work goes to this stream (hipStream_t as void*, not NULL: graphs are
+ * captured on it). By default the batch owns a non-blocking stream. */
+int achr_batch_set_stream(achr_batch *batch, void *hip_stream);
+
+/* Autotuner: for every size class (n in [2^k, 2^(k+1))) times each candidate
+ * threads-per-chain on probe_steps steps of P[k] chains of every model of the
+ * class and keeps the fastest. ms, if not NULL, receives count x 7 probe times
+ * (candidates 16, 32, ..., 1024 lanes; -1 where not a candidate). Chain
+ * states of the probe are discarded. */
+int achr_batch_autotune(achr_batch *batch, const int *P, int probe_steps, uint64_t seed, double *ms);
+
+/* Threads per chain of every model (count ints); set_lanes pins them
+ * (0 = the sampler's own schedule). */
+int achr_batch_lanes(const achr_batch *batch, int *lanes);
+int achr_batch_set_lanes(achr_batch *batch, const int *lanes);
+
+/* One file of every model: P[k] chains of model k, S accepted steps each.
+ * use_graph = 0 launches the same work eagerly. Returns when the run ends. */
+int achr_batch_sample(achr_batch *batch, const int *P, int S, uint64_t seed, uint32_t file, int use_graph);
+
+/* Milliseconds of the last run (events around all its work), its segments,
+ * buckets (persistent launches per segment), distinct graphs and their
+ * total node count. */
+int achr_batch_timing(const achr_batch *batch, double *ms_total, int *segments, int *buckets, int *graphs,
+                      int *graph_nodes);
 
 /* ---------------------------------------- learned direction distribution */
 
