@@ -169,21 +169,39 @@ def free_directions(N: np.ndarray, lb: np.ndarray, ub: np.ndarray) -> np.ndarray
 
 
 def chord_warmup(N: np.ndarray, x0: np.ndarray, lb: np.ndarray, ub: np.ndarray, W: int, seed: int,
-                 shrink: float = 0.999) -> np.ndarray:
+                 shrink: float = 0.999, free: np.ndarray | None = None, mix: int = 0) -> np.ndarray:
     """W points on random chords through x0 along directions that keep S x = 0
-    and the fixed reactions fixed, then moved to the centre."""
+    and the fixed reactions fixed, then moved to the centre.
+
+    Directions are Gaussian combinations of the free null-space basis (`free`,
+    computed from N when not given): of every basis vector (mix = 0), or of
+    `mix` randomly chosen ones (O(n W mix) work for large models)."""
     rng = np.random.default_rng(seed)
-    N = free_directions(N, lb, ub)
+    N = free_directions(N, lb, ub) if free is None else free
+    if mix > 0:
+        N = np.asfortranarray(N)  # column gathers
     n, d = N.shape
     out = np.empty((n, W))
     B = 512
     for k0 in range(0, W, B):
         k1 = min(W, k0 + B)
-        G = rng.standard_normal((d, k1 - k0))
-        V = N @ G
+        if mix > 0:
+            idx = rng.integers(0, d, size=(k1 - k0, mix))
+            coef = rng.standard_normal((k1 - k0, mix))
+            V = np.zeros((n, k1 - k0))
+            for j in range(mix):
+                V += N[:, idx[:, j]] * coef[:, j]
+        else:
+            G = rng.standard_normal((d, k1 - k0))
+            V = N @ G
+        pos = V > 1e-12
+        flat = ~(pos | (V < -1e-12))
+        a, b = (ub - x0)[:, None], (lb - x0)[:, None]
         with np.errstate(divide="ignore", invalid="ignore"):
-            up = np.where(V > 1e-12, (ub - x0)[:, None] / V, np.where(V < -1e-12, (lb - x0)[:, None] / V, np.inf))
-            dn = np.where(V > 1e-12, (lb - x0)[:, None] / V, np.where(V < -1e-12, (ub - x0)[:, None] / V, -np.inf))
+            up = np.where(pos, a, b) / V
+            dn = np.where(pos, b, a) / V
+        up[flat] = np.inf
+        dn[flat] = -np.inf
         tmax = up.min(axis=0)
         tmin = dn.max(axis=0)
         far = (np.arange(k0, k1) % 2) == 1

@@ -175,9 +175,11 @@ struct Row {
 };
 
 // One segment of chain b of the model `a` (b is group-uniform; with TPC > 64
-// the block is one chain, so every early return is block-uniform).
+// the block is one chain, so every early return is block-uniform). The
+// arguments come by value: a reference to the kernel's argument block keeps
+// the compiler from scalarising it and costs about 40 more spilled VGPRs.
 template <int TPC, int E, int CPB, bool DIR>
-__device__ __forceinline__ void run_chain(const StepArgs& a, const int b, double* lds) {
+__device__ __forceinline__ void run_chain(const StepArgs a, const int b, double* lds) {
   double* red = lds;  // 4 reduction slot arrays; only block-per-chain schedules use them
   const int g = threadIdx.x / TPC;
   const int lane = threadIdx.x % TPC;
@@ -238,8 +240,8 @@ __device__ __forceinline__ void run_chain(const StepArgs& a, const int b, double
   bool stopped_at_check = false;
   const uint32_t chain = static_cast<uint32_t>(a.chain0 + b);
 
-  // Bank mode: the direction of attempt a + 1 is loaded while attempt a runs
-  // (the pick is a function of the attempt counter only).
+  // Bank mode: the direction of attempt a + 1 is loaded while attempt a
+  // finishes (the pick is a function of the attempt counter only).
   double wn[DIR ? E : 1];
   auto load_dir = [&](uint32_t att) {
     const int idn = pick_index(stream_u01(a.seed, att, chain, a.file, a.model, kTagPick), a.W);
@@ -268,8 +270,6 @@ __device__ __forceinline__ void run_chain(const StepArgs& a, const int b, double
       if constexpr (DIR) {
 #pragma unroll
         for (int k = 0; k < E; ++k) u[k] = wn[k];
-        ++attempt;
-        load_dir(attempt);
       } else {
         const double up = stream_u01(a.seed, attempt, chain, a.file, a.model, kTagPick);
         ++attempt;
@@ -312,9 +312,18 @@ __device__ __forceinline__ void run_chain(const StepArgs& a, const int b, double
         const double dl = x[k] - bload(r_lb, off);
         // Both branches of the reference (ACHRCuda.cu:270-279) divide the same
         // two distances by u_i; divide once and select, so a wavefront whose
-        // lanes disagree on the sign does not run both sets of divisions.
-        const double r_lo = -dl / ui;
-        const double r_hi = du / ui;
+        // lanes disagree on the sign does not run both sets of divisions. A
+        // bank direction (no reference counterpart) takes one reciprocal of
+        // u_i and two products instead of two divisions.
+        double r_lo, r_hi;
+        if constexpr (DIR) {
+          const double rcp = 1.0 / ui;
+          r_lo = -dl * rcp;
+          r_hi = du * rcp;
+        } else {
+          r_lo = -dl / ui;
+          r_hi = du / ui;
+        }
         const bool keep = du > a.dTol && dl > a.dTol;
         const bool pos = keep && ui > a.uTol;
         const bool neg = keep && ui < -a.uTol;
@@ -328,6 +337,13 @@ __device__ __forceinline__ void run_chain(const StepArgs& a, const int b, double
         if constexpr (E > 4) {
           if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
+      }
+      if constexpr (DIR) {
+        // Prefetch the next attempt's direction once the bounds loop no
+        // longer needs its temporaries; it lands during the reduction, the
+        // advance and the clip.
+        ++attempt;
+        load_dir(attempt);
       }
       double* rmm = red + ((DIR && (attempt & 1u)) ? kRedMinMax1 : kRedMinMax0) * kRedSlots;
       group_minmax<TPC>(mn, mx, rmm, rmm + kRedSlots);

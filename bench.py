@@ -155,12 +155,182 @@ def roofline_for(args_proj, m, n, d, P, S, q, tm):
                           f"{attempts} attempts, {segs} launches"}
 
 
+def gpu_nullspace(rows, cols, vals, m, n):
+    import achr
+    mdl = achr.Model.from_coo(n, m, rows, cols, vals, -np.ones(n), np.ones(n))
+    mdl.compute_nullspace()
+    return mdl.nullspace()
+
+
+def batch_model(n, seed):
+    """One model of the batch workload: m = 0.55 n metabolites, the sparsity
+    law of the 72 x 95 model, 30% irreversible, 2% fixed, bounds x0 +- (5..50).
+    Returns it with the basis of its free null space (GPU null spaces)."""
+    from achr import synth
+    mm = int(round(0.55 * n))
+    sm = synth.make_model(mm, n, seed, irrev_frac=0.30, n_fixed=max(1, int(round(0.02 * n))), width=(5.0, 50.0),
+                          x0_scale=10.0, nullspace_fn=gpu_nullspace)
+    fixed = np.nonzero(sm.lb == sm.ub)[0]
+    # free directions: null([S; e_i for the fixed reactions i])
+    r = np.concatenate([sm.rows, mm + np.arange(len(fixed))]).astype(np.int32)
+    c = np.concatenate([sm.cols, fixed]).astype(np.int32)
+    v = np.concatenate([sm.vals, np.ones(len(fixed))])
+    return sm, gpu_nullspace(r, c, v, mm + len(fixed), n)
+
+
+def batch_warmup(sm, Nf, seed):
+    """2n warmup points on free null-space chords (exact FP64; each chord
+    direction mixes 8 free basis vectors)."""
+    from achr import synth
+    return synth.chord_warmup(None, sm.x0, sm.lb, sm.ub, 2 * sm.n, seed=seed, free=Nf, mix=8)
+
+
+def csr_of_model(sm):
+    order = np.lexsort((sm.cols, sm.rows))
+    rows, cols, vals = sm.rows[order], sm.cols[order], sm.vals[order]
+    rp = np.zeros(sm.m + 1, dtype=np.int64)
+    np.add.at(rp, rows + 1, 1)
+    return np.cumsum(rp).astype(np.int32), cols.astype(np.int32), vals
+
+
+def run_batch(args):
+    """64 models, n log-uniform in [100, 13000], 2,000 chains x 2,000 steps
+    each, in one batch: autotuned lanes per size class, one persistent launch
+    per schedule bucket and segment, segments replayed from HIP graphs."""
+    import achr
+    import scipy.sparse as sp
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    count, P, S, seed = args.models, args.chains or 2000, args.chain_steps or 2000, 3
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(100), np.log(13000), count)).round().astype(int)
+    t0 = time.time()
+    models, smps, info, cpu_sample = [], [], [], []
+    cpu_t, cpu_steps = 0.0, 0
+    threads = min(16, os.cpu_count() or 1)
+    built = []
+    for k, n in enumerate(ns):
+        built.append(batch_model(int(n), seed * 1000 + k))
+    log(f"{count} models and free null spaces in {time.time() - t0:.1f}s")
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        futs = [pool.submit(batch_warmup, sm, Nf, seed * 1000 + k + 1) for k, (sm, Nf) in enumerate(built)]
+        warms = [f.result() for f in futs]
+    log(f"warmups in {time.time() - t0:.1f}s")
+    for k, (sm, _) in enumerate(built):
+        W = warms[k]
+        warms[k] = None
+        mdl = achr.Model.from_coo(sm.n, sm.m, sm.rows, sm.cols, sm.vals, sm.lb, sm.ub)
+        mdl.set_nullspace(sm.N)
+        smp = achr.Sampler(mdl, W, achr.Options(model_id=k))
+        if not args.no_cpu:
+            rp, ci, va = csr_of_model(sm)
+            t = time.time()
+            O.sample(warm=W, lb=sm.lb, ub=sm.ub, rowptr=rp, colidx=ci, vals=va, N=sm.N, m=sm.m, P=args.cpu_chains,
+                     S=args.cpu_steps, seed=1, model=k, lanes=smp.schedule()[0], threads=threads)
+            cpu_t += time.time() - t
+            cpu_steps += args.cpu_chains * args.cpu_steps
+        models.append((sm, mdl))
+        smps.append(smp)
+        info.append({"n": int(sm.n), "m": int(sm.m), "d": int(sm.N.shape[1])})
+        del W
+        if k % 8 == 7:
+            log(f"{k + 1}/{count} samplers in {time.time() - t0:.1f}s")
+    del built
+    t_build = time.time() - t0
+    batch = achr.Batch(smps)
+    t = time.time()
+    probe = batch.autotune([P] * count, probe_steps=args.probe_steps, seed=1)
+    t_tune = time.time() - t
+    lanes = batch.lanes()
+    classes = {}
+    for k in range(count):
+        cl = int(np.floor(np.log2(info[k]["n"])))
+        classes.setdefault(cl, {"n_range": [info[k]["n"], info[k]["n"]], "models": 0, "lanes": int(lanes[k]),
+                                "probe_ms": {str(achr.Batch.TPC[j]): round(float(probe[k, j]), 3)
+                                             for j in range(7) if probe[k, j] > 0}})
+        c = classes[cl]
+        c["models"] += 1
+        c["n_range"] = [min(c["n_range"][0], info[k]["n"]), max(c["n_range"][1], info[k]["n"])]
+    log(f"autotune {t_tune:.1f}s: {classes}")
+    for i in range(args.warmup):
+        batch.run([P] * count, min(S, 20), seed=1000 + i)
+    walls, tms = [], []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        batch.run([P] * count, S, seed=seed + i, file=i)
+        walls.append(time.perf_counter() - t)
+        tms.append(batch.timing())
+        log(f"batch file {i}: {walls[-1]:.3f}s wall, {tms[-1]}")
+    j = int(np.argmax(walls))
+    wall, tm = walls[j], tms[j]
+    total_steps = count * P * S
+    pts_b = [s.points() for s in smps]
+    st_b = [s.stats() for s in smps]
+    # the same models one at a time (the sampler's own launches), another seed
+    t = time.perf_counter()
+    for s in smps:
+        s.run(P, S, seed=seed + 100)
+    for s in smps:
+        s.timing()
+    wall_solo = time.perf_counter() - t
+    log(f"solo runs: {wall_solo:.3f}s")
+    dmean, dstd, zmean, drift = [], [], [], []
+    for k, s in enumerate(smps):
+        sm = models[k][0]
+        pb, ps = pts_b[k], s.points()
+        rng_ = sm.ub - sm.lb
+        live = rng_ > 0
+        dm = np.abs(pb.mean(1) - ps.mean(1))[live]
+        se = np.sqrt((pb.var(1) + ps.var(1))[live] / P)
+        dmean.append(float((dm / rng_[live]).max()))
+        dstd.append(float((np.abs(pb.std(1) - ps.std(1))[live] / rng_[live]).max()))
+        zmean.append(float((dm / np.maximum(se, 1e-300)).max()))
+        Ssp = sp.csr_matrix((sm.vals, (sm.rows, sm.cols)), shape=(sm.m, sm.n))
+        drift.append(float(np.abs(Ssp @ pb).max()))
+    attempts = sum(int(st[:, 0].sum()) + P * S for st in st_b)
+    reproj = sum(int(st[:, 1].sum()) for st in st_b)
+    sum_n = sum(i["n"] for i in info)
+    step_bytes = sum(8.0 * i["n"] * (P * S + int(st_b[k][:, 0].sum())) + 32.0 * i["n"] * P * (tm["segments"] - 1)
+                     for k, i in enumerate(info))
+    ach = step_bytes / (tm["ms_total"] * 1e-3) / 1e9
+    out = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "metric": f"aggregate chain-steps/s ({count}-model batch, plain FP64 ACHR)",
+           "value": total_steps / wall, "unit": "chain-steps/s", "ms_per_step": wall * 1e3,
+           "config": {"workload": f"batch: {count} synthetic models, n log-uniform in [100, 13000] "
+                                  f"(sum {sum_n}), m = 0.55 n, {P} chains x {S} steps each, 2n exact warmup points",
+                      "models": count, "chains_per_model": P, "chain_steps": S, "reactions_sum": sum_n,
+                      "reactions_min": int(min(ns)), "reactions_max": int(max(ns))},
+           "batch": {**tm, "autotune_s": t_tune, "build_s": t_build, "size_classes": classes},
+           "solo": {"s_total": wall_solo, "chain_steps_per_s": total_steps / wall_solo,
+                    "batch_speedup": wall_solo / wall},
+           "moment_agreement_vs_solo_fp64": {
+               "mean_frac_max": max(dmean), "mean_frac_median_model": float(np.median(dmean)),
+               "std_frac_max": max(dstd), "std_frac_median_model": float(np.median(dstd)),
+               "mean_z_max": max(zmean), "note": "per model: max over reactions of |batch - solo| / (ub - lb), "
+                                                  "independent seeds; z = |d mean| / its standard error"},
+           "drift_inf_max": max(drift), "reprojections": reproj, "attempts": attempts,
+           "roofline": {"bound": "hbm", "kernel": "step_batch_kernel (whole graph-replayed run)", "achieved": ach,
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                        "per_launch": "8n B per attempt + 32n B per chain per resumed segment, summed over models; "
+                                      "time = the whole run including projection rounds"}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = {"value": cpu_steps / cpu_t, "unit": "chain-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"oracle (C, OpenMP) {args.cpu_chains} chains x {args.cpu_steps} steps of "
+                                         f"each of the {count} models in {cpu_t:.1f}s"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed files (bench steps)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed files (short: 20 chain steps)")
-    ap.add_argument("--workload", choices=["genome", "ecoli"], default="genome")
+    ap.add_argument("--workload", choices=["genome", "ecoli", "batch"], default="genome")
+    ap.add_argument("--models", type=int, default=64, help="batch workload: number of models")
+    ap.add_argument("--probe-steps", type=int, default=20, help="batch workload: autotuner probe length")
     ap.add_argument("--sampler", choices=["learned", "plain"], default="learned")
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--chain-steps", type=int, default=None)
@@ -184,6 +354,12 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (chains shard independently across "
               f"GPUs, see the design notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        if args.cpu_chains == 32 and args.cpu_steps == 300:
+            args.cpu_chains, args.cpu_steps = 16, 40
+        run_batch(args)
+        return
 
     import achr
     from achr import stats as st

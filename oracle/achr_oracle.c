@@ -261,14 +261,22 @@ static void run_chain(const oracle_model *M, const oracle_opts *o, int P, int S,
       const double ui = M->bank ? u[i] : u[i] / nrm;
       u[i] = ui;
       const double du = M->ub[i] - x[i], dl = x[i] - M->lb[i];
-      if (du > o->dTol && dl > o->dTol) {
-        if (ui > o->uTol) {
-          double v = -dl / ui; if (v > mn) mn = v;
-          v = du / ui; if (v < mx) mx = v;
-        } else if (ui < -o->uTol) {
-          double v = du / ui; if (v > mn) mn = v;
-          v = -dl / ui; if (v < mx) mx = v;
+      if (du > o->dTol && dl > o->dTol && (ui > o->uTol || ui < -o->uTol)) {
+        /* reference (ACHRCuda.cu:270-279): the two distances divided by u_i;
+         * bank mode (no reference counterpart): one reciprocal of u_i, then
+         * two products */
+        double lo, hi;
+        if (M->bank) {
+          const double rcp = 1.0 / ui;
+          lo = -dl * rcp;
+          hi = du * rcp;
+        } else {
+          lo = -dl / ui;
+          hi = du / ui;
         }
+        if (ui < 0.0) { const double tmp = lo; lo = hi; hi = tmp; }
+        if (lo > mn) mn = lo;
+        if (hi < mx) mx = hi;
       }
     }
     int miss = !(nrm > 0.0) || !(mn > -INFINITY) || !(mx < INFINITY) ||
