@@ -32,6 +32,7 @@ namespace achr {
 namespace {
 
 constexpr int kRedSlots = 16;  // wavefront partials per reduction (<= 1024 threads)
+constexpr int kPrefetchMaxE = 8;  // largest slot count that prefetches warmup points
 // LDS slot arrays of the block-per-chain reductions: the norm, two pairs for
 // min/max and the residual maximum. Consecutive reductions must use disjoint
 // arrays (a wavefront may write the next partial while a slower one still
@@ -240,10 +241,14 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
   bool stopped_at_check = false;
   const uint32_t chain = static_cast<uint32_t>(a.chain0 + b);
 
-  // Bank mode: the direction of attempt a + 1 is loaded while attempt a
-  // finishes (the pick is a function of the attempt counter only).
-  double wn[DIR ? E : 1];
+  // The warmup point (or bank direction) of attempt a + 1 is loaded while
+  // attempt a finishes: the pick is a function of the attempt counter only.
+  // Without a bank, schedules of more than 8 slots load the point at the
+  // top of the step instead (the prefetch registers would spill).
+  constexpr bool kPre = DIR || E <= kPrefetchMaxE;
+  double wn[kPre ? E : 1];
   auto load_dir = [&](uint32_t att) {
+    if constexpr (!kPre) return;
     const int idn = pick_index(stream_u01(a.seed, att, chain, a.file, a.model, kTagPick), a.W);
     if constexpr (TPC >= 64) {
       const int idu = __builtin_amdgcn_readfirstlane(idn);
@@ -259,41 +264,42 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
       }
     }
   };
-  if constexpr (DIR) load_dir(attempt);
+  load_dir(attempt);
 
   for (;;) {
     if (!pending) {
       if (s >= a.S) break;
       double u[E];
       double ss = 0.0;
-      int id = 0;
-      if constexpr (DIR) {
-#pragma unroll
-        for (int k = 0; k < E; ++k) u[k] = wn[k];
-      } else {
-        const double up = stream_u01(a.seed, attempt, chain, a.file, a.model, kTagPick);
-        ++attempt;
-        id = pick_index(up, a.W);
-      }
-      if constexpr (DIR) {
-      } else if constexpr (TPC >= 64) {
-        // id is uniform over the wavefront: a per-column resource in SGPRs.
-        const int idu = __builtin_amdgcn_readfirstlane(id);
-        const rsrc_t r_w = make_rsrc(a.warm + static_cast<size_t>(idu) * n, 8LL * n);
+      if constexpr (kPre) {
 #pragma unroll
         for (int k = 0; k < E; ++k) {
-          const double wk = bload(r_w, (lane + k * TPC) * 8);
-          u[k] = wk - (kCL ? cl[lane + k * TPC] : c[k]);
-          ss = fma(u[k], u[k], ss);
+          if constexpr (DIR) {
+            u[k] = wn[k];
+          } else {
+            u[k] = wn[k] - (kCL ? cl[lane + k * TPC] : c[k]);
+            ss = fma(u[k], u[k], ss);
+          }
         }
       } else {
-        const int wbase = id * n;
+        const int id = pick_index(stream_u01(a.seed, attempt, chain, a.file, a.model, kTagPick), a.W);
+        if constexpr (TPC >= 64) {
+          // id is uniform over the wavefront: a per-column resource in SGPRs.
+          const int idu = __builtin_amdgcn_readfirstlane(id);
+          const rsrc_t r_w = make_rsrc(a.warm + static_cast<size_t>(idu) * n, 8LL * n);
 #pragma unroll
-        for (int k = 0; k < E; ++k) {
-          const int i = lane + k * TPC;
-          const double wk = bload(r_wall, i < n ? (wbase + i) * 8 : kOob);
-          u[k] = wk - (kCL ? cl[i] : c[k]);
-          ss = fma(u[k], u[k], ss);
+          for (int k = 0; k < E; ++k) {
+            u[k] = bload(r_w, (lane + k * TPC) * 8) - (kCL ? cl[lane + k * TPC] : c[k]);
+            ss = fma(u[k], u[k], ss);
+          }
+        } else {
+          const int wbase = id * n;
+#pragma unroll
+          for (int k = 0; k < E; ++k) {
+            const int i = lane + k * TPC;
+            u[k] = bload(r_wall, i < n ? (wbase + i) * 8 : kOob) - (kCL ? cl[i] : c[k]);
+            ss = fma(u[k], u[k], ss);
+          }
         }
       }
       // Bank directions are unit vectors: no norm, no division.
@@ -338,18 +344,13 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
           if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
       }
-      if constexpr (DIR) {
-        // Prefetch the next attempt's direction once the bounds loop no
-        // longer needs its temporaries; it lands during the reduction, the
-        // advance and the clip.
-        ++attempt;
-        load_dir(attempt);
-      }
       double* rmm = red + ((DIR && (attempt & 1u)) ? kRedMinMax1 : kRedMinMax0) * kRedSlots;
       group_minmax<TPC>(mn, mx, rmm, rmm + kRedSlots);
       const bool miss = !(nrm > 0.0) || !(mn > -INFINITY) || !(mx < INFINITY) ||
                         (fabs(mn) < a.maxMinTol && fabs(mx) < a.maxMinTol) || (mn > mx);
       if (miss) {
+        ++attempt;
+        load_dir(attempt);
         ++sts.x;
         if (++consec > a.max_misses) {
           sts.w |= 1;
@@ -362,6 +363,11 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
       const double t = fma(rs, mx - mn, mn);
 #pragma unroll
       for (int k = 0; k < E; ++k) x[k] = fma(t, u[k], x[k]);
+      // Prefetch the next attempt's point once this direction is consumed
+      // (never two of them in registers); it lands during the residual
+      // check, the clip, the centre update and the next step's set-up.
+      ++attempt;
+      load_dir(attempt);
 
       if (s % a.check_every == 0) {
         ++sts.z;

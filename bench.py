@@ -1,10 +1,16 @@
-"""Benchmark: min-ESS/sec over reactions of plain FP64 ACHR on one MI355X.
+"""Benchmark: min-ESS/sec over reactions on one MI355X.
 
-Workload (default "genome"): the 5,800 x 10,600 synthetic genome-scale model
+Default workload ("genome"): the 5,800 x 10,600 synthetic genome-scale model
 (3.2 nonzeros per column, 40% irreversible, 2% fixed, bounds +-(100..1000)
 around an interior null-space point), 21,200 warmup points stored with the
 6 decimals of the reference's "%f" warmup CSV, 20,000 chains x 5,000 accepted
-steps. "ecoli" is the 72 x 95 fixture model, 500 chains x 1,000 steps.
+steps. The default sampler is the learned one: a plain-ACHR training run
+(1,000 steps, every chain traced), the direction network trained on those
+200,000 samples, then the timed run along the learned direction bank; the
+plain FP64 sampler runs the same file for comparison (speedup, moment
+agreement, split-R-hat). "ecoli" is the 72 x 95 fixture model, 500 chains x
+1,000 steps; "batch" is 64 models of 100..13,000 reactions in one job
+(aggregate chain-steps/s).
 
 One bench step = one file of the reference CLI (all chains, all steps), with
 inputs already resident in HBM. The timed region is bracketed by device
@@ -128,11 +134,16 @@ def sample_quality(m, smp, S, trace_stride_chains, P):
             "traced": int(trace.shape[0]), "drift": drift, "max_rhat": float(np.nanmax(rhat))}
 
 
-def roofline_for(args_proj, m, n, d, P, S, q, tm):
+def roofline_for(args_proj, m, n, d, P, S, q, tm, bank=False):
+    """Roofline of the dominant kernel. Step kernel: 8n B per attempt (the
+    gathered warmup point or bank direction) plus, per chain and resumed
+    segment, the state read and written back: x and the centre (32n B), or
+    x alone in bank mode (16n B)."""
     reproj = int(q["stats"][:, 1].sum())
     attempts = P * S + int(q["stats"][:, 0].sum())
     segs = tm["step_launches"]
-    step_bytes = 8.0 * n * attempts + 32.0 * n * P * max(segs - 1, 0)
+    state = 16.0 if bank else 32.0
+    step_bytes = 8.0 * n * attempts + state * n * P * max(segs - 1, 0)
     step_ms, proj_ms = tm["ms_step"], tm["ms_proj"]
     if args_proj == "fp64":
         proj_flops = 4.0 * n * d * reproj
@@ -151,7 +162,7 @@ def roofline_for(args_proj, m, n, d, P, S, q, tm):
     ach = step_bytes / (step_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": "step_kernel", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS, "traffic": None, "launches": segs,
-            "per_launch": f"8n B per attempt (direction gather) + 32n B per chain per resumed segment; "
+            "per_launch": f"8n B per attempt (direction gather) + {int(state)}n B per chain per resumed segment; "
                           f"{attempts} attempts, {segs} launches"}
 
 
@@ -436,16 +447,29 @@ def main():
                     "drift_inf": q["drift"], "max_rhat": q["max_rhat"],
                     "learn": {"train_run_s": t_train_run, "learn_s": t_learn, **rep},
                     "kernel_ms": tm})
-        out["roofline"] = roofline_for("fp64", m, n, d, P, S, q, tm)
+        out["roofline"] = roofline_for("fp64", m, n, d, P, S, q, tm, bank=True)
         if plain_q is not None:
             rng = m.ub - m.lb
             live = rng > 0
-            dm = np.abs(q["pts"].mean(1) - plain_q["pts"].mean(1))[live] / rng[live]
-            ds = np.abs(q["pts"].std(1) - plain_q["pts"].std(1))[live] / rng[live]
+            A, B = q["pts"][live], plain_q["pts"][live]
+            dm_abs = np.abs(A.mean(1) - B.mean(1))
+            dm = dm_abs / rng[live]
+            ds = np.abs(A.std(1) - B.std(1)) / rng[live]
+            # chain endpoints are independent: standard errors of the two
+            # differences (the std's by the normal approximation)
+            se_m = np.sqrt((A.var(1) + B.var(1)) / P)
+            se_s = np.sqrt((A.var(1) + B.var(1)) / (2 * P))
+            dsd_abs = np.abs(A.std(1) - B.std(1))
             out["plain_fp64"] = plain
             out["speedup_vs_plain"] = value / plain["min_ess_per_s"]
-            out["moment_err_vs_plain"] = {"mean_frac_max": float(dm.max()), "std_frac_max": float(ds.max()),
-                                          "mean_frac_median": float(np.median(dm))}
+            out["moment_err_vs_plain"] = {
+                "mean_frac_max": float(dm.max()), "std_frac_max": float(ds.max()),
+                "mean_frac_median": float(np.median(dm)), "std_frac_median": float(np.median(ds)),
+                "mean_z_max": float((dm_abs / np.maximum(se_m, 1e-300)).max()),
+                "std_z_max": float((dsd_abs / np.maximum(se_s, 1e-300)).max()),
+                "z_max_of_pure_noise_approx": float(np.sqrt(2.0 * np.log(int(live.sum())))),
+                "note": "|learned - plain| over the live reactions, as a fraction of ub - lb and in standard errors "
+                        "of the difference (P independent chain endpoints each)"}
         ess_per_step_ref = plain_q["ess_per_step"] if plain_q is not None else None
     if not args.no_cpu and ess_per_step_ref is not None:
         threads = min(16, os.cpu_count() or 1)

@@ -126,3 +126,27 @@ def test_model_create_validates():
         achr.Model.from_coo(3, 2, [0, 5], [0, 1], [1.0, 1.0], np.zeros(3), np.ones(3))
     m = achr.Model.from_coo(3, 2, [0, 0, 1, 0], [0, 1, 2, 0], [1.0, 2.0, 3.0, 1.0], np.zeros(3), np.ones(3))
     np.testing.assert_array_equal(m.dense(), [[2, 2, 0], [0, 0, 3]])  # duplicates summed
+
+
+def test_batch_rejects_bad_arguments_without_touching_the_device():
+    L = achr.lib()
+    h = ctypes.c_void_p()
+    assert L.achr_batch_create(None, 0, ctypes.byref(h)) == -1
+    two_null = (ctypes.c_void_p * 2)(None, None)
+    assert L.achr_batch_create(two_null, 2, ctypes.byref(h)) == -1
+    assert b"NULL" in L.achr_last_error()
+    assert L.achr_batch_create(two_null, 2, None) == -1
+    P = (ctypes.c_int * 2)(10, 10)
+    assert L.achr_batch_sample(None, P, 10, 0, 0, 1) == -1
+    assert L.achr_batch_autotune(None, P, 10, 0, None) == -1
+    assert L.achr_batch_lanes(None, P) == -1
+    assert L.achr_batch_set_lanes(None, P) == -1
+    assert L.achr_batch_set_stream(None, None) == -1
+    assert L.achr_batch_timing(None, None, None, None, None, None) == -1
+    L.achr_batch_destroy(None)
+
+
+def test_options_model_id_default_and_range():
+    o = _lib.AchrOptions()
+    achr.lib().achr_options_default(ctypes.byref(o))
+    assert o.model_id == 0 and o.fp64_every == 10
