@@ -74,6 +74,7 @@ SIGNATURES = {
     "achr_sampler_learn_directions": (_I, [_P, _P, _PD]),
     "achr_sampler_set_directions": (_I, [_P, _PD, _I]),
     "achr_sampler_use_directions": (_I, [_P, _I]),
+    "achr_sampler_get_directions": (_I, [_P, _PD, _PI]),
     "achr_sampler_set_run_options": (_I, [_P, _I, _I, _I]),
     "achr_batch_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
     "achr_batch_destroy": (None, [_P]),

@@ -237,6 +237,15 @@ class Sampler:
         flat = np.ascontiguousarray(b.ravel(order="F"))
         check(lib().achr_sampler_set_directions(self._h, _dp(flat), b.shape[1]), "achr_sampler_set_directions")
 
+    def directions(self) -> np.ndarray:
+        """The sampler's direction bank (n x K), empty if it has none."""
+        K = C.c_int()
+        check(lib().achr_sampler_get_directions(self._h, None, C.byref(K)), "achr_sampler_get_directions")
+        out = np.empty((self.n, K.value), order="F")
+        if K.value:
+            check(lib().achr_sampler_get_directions(self._h, _dp(out), C.byref(K)), "achr_sampler_get_directions")
+        return out
+
     def set_run_options(self, trace_chains: int, trace_thin: int, check_every: int) -> None:
         check(lib().achr_sampler_set_run_options(self._h, trace_chains, trace_thin, check_every),
               "achr_sampler_set_run_options")

@@ -171,6 +171,14 @@ int achr_sampler_set_directions(achr_sampler* S, const double* bank, int K) {
   return ACHR_OK;
 }
 
+int achr_sampler_get_directions(const achr_sampler* S, double* bank, int* K) {
+  if (!S || !K) return fail(ACHR_E_ARG, "NULL argument");
+  *K = S->K;
+  if (bank && S->K > 0)
+    HIP_TRY(hipMemcpy(bank, S->dirs.p, sizeof(double) * static_cast<size_t>(S->n) * S->K, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
 int achr_sampler_set_run_options(achr_sampler* S, int trace_chains, int trace_thin, int check_every) {
   if (!S || trace_chains < 0 || (trace_chains > 0 && trace_thin <= 0) || check_every <= 0)
     return fail(ACHR_E_ARG, "bad run options");

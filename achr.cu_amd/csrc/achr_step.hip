@@ -92,15 +92,16 @@ __device__ __forceinline__ double group_sum(double v, double* red) {
   return v;
 }
 
+// Maxima and minima of values that are never NaN (chord ends, +-infinity,
+// absolute residuals): fmax / fmin give the same bits as compare-and-assign
+// and are single instructions.
 template <int TPC>
 __device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red_mn, double* red_mx) {
   constexpr int G = TPC < 64 ? TPC : 64;
 #pragma unroll
   for (int o = G / 2; o >= 1; o >>= 1) {
-    const double a = __shfl_xor(mn, o);
-    const double b = __shfl_xor(mx, o);
-    if (a > mn) mn = a;
-    if (b < mx) mx = b;
+    mn = fmax(mn, __shfl_xor(mn, o));
+    mx = fmin(mx, __shfl_xor(mx, o));
   }
   if constexpr (TPC > 64) {
     constexpr int NW = TPC / 64;
@@ -111,9 +112,8 @@ __device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
-      const double a = red_mn[i], b = red_mx[i];
-      if (a > mn) mn = a;
-      if (b < mx) mx = b;
+      mn = fmax(mn, red_mn[i]);
+      mx = fmin(mx, red_mx[i]);
     }
   }
 }
@@ -122,19 +122,13 @@ template <int TPC>
 __device__ __forceinline__ double group_max(double v, double* red) {
   constexpr int G = TPC < 64 ? TPC : 64;
 #pragma unroll
-  for (int o = G / 2; o >= 1; o >>= 1) {
-    const double a = __shfl_xor(v, o);
-    if (a > v) v = a;
-  }
+  for (int o = G / 2; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
   if constexpr (TPC > 64) {
     constexpr int NW = TPC / 64;
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < NW; ++i) {
-      const double a = red[i];
-      if (a > v) v = a;
-    }
+    for (int i = 0; i < NW; ++i) v = fmax(v, red[i]);
   }
   return v;
 }
@@ -330,13 +324,17 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
           r_lo = -dl / ui;
           r_hi = du / ui;
         }
-        const bool keep = du > a.dTol && dl > a.dTol;
-        const bool pos = keep && ui > a.uTol;
-        const bool neg = keep && ui < -a.uTol;
-        const double lo = pos ? r_lo : (neg ? r_hi : -INFINITY);
-        const double hi = pos ? r_hi : (neg ? r_lo : INFINITY);
-        if (lo > mn) mn = lo;
-        if (hi < mx) mx = hi;
+        // A kept coordinate has both distances > dTol > 0, so the two
+        // quotients have opposite signs: the lower candidate is their
+        // minimum and the upper their maximum whatever the sign of u_i
+        // (the reference's two branches pick exactly these). Kept
+        // candidates are never NaN, so fmax / fmin equal its compare-and-
+        // assign; coordinates that are not kept contribute nothing.
+        const bool use = du > a.dTol && dl > a.dTol && fabs(ui) > a.uTol;
+        const double lo = fmin(r_lo, r_hi);
+        const double hi = fmax(r_lo, r_hi);
+        mn = use ? fmax(mn, lo) : mn;
+        mx = use ? fmin(mx, hi) : mx;
         // Schedule four slots at a time: their bound loads overlap, while
         // hoisting every slot's loads and FP64 divisions ahead would multiply
         // the live registers by E.

@@ -334,6 +334,46 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+_MODEL_FIELDS = ("rows", "cols", "vals", "lb", "ub", "x0", "N")
+
+
+def save_inputs(path, m, W):
+    os.makedirs(path, exist_ok=True)
+    for f in _MODEL_FIELDS:
+        np.save(os.path.join(path, f + ".npy"), getattr(m, f))
+    np.save(os.path.join(path, "dims.npy"), np.array([m.m, m.n]))
+    np.save(os.path.join(path, "W.npy"), W)
+
+
+def load_inputs(path):
+    from achr import synth
+    v = {f: np.load(os.path.join(path, f + ".npy")) for f in _MODEL_FIELDS}
+    mm, n = (int(x) for x in np.load(os.path.join(path, "dims.npy")))
+    m = synth.SynthModel(None, v["rows"], v["cols"], v["vals"], mm, n, v["lb"], v["ub"], v["x0"], v["N"])
+    log(f"inputs loaded from {path}")
+    return m, np.load(os.path.join(path, "W.npy"))
+
+
+def calibrate_copy(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime (a
+    blit kernel reading and writing exactly that many bytes)."""
+    import ctypes as C
+    import achr
+    achr.lib()  # loads the HIP runtime the library links
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+    rc = hip.hipMemset(a, 1, nbytes) or hip.hipMemcpy(b, a, nbytes, 3) or hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc:
+        raise RuntimeError(f"calibration copy failed ({rc})")
+    log(f"calibration copy: {mib} MiB device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +398,11 @@ def main():
     ap.add_argument("--check-every", type=int, default=100, help="residual check period of the learned sampler")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 ACHR comparison run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--save-inputs", default=None, help="genome workload: save model, warmup and bank (.npy) here")
+    ap.add_argument("--load-inputs", default=None,
+                    help="genome workload: load them from here instead (no solver or training in the process)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this size (a known-traffic kernel for counters)")
     ap.add_argument("--cpu-chains", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=300)
     args = ap.parse_args()
@@ -366,6 +411,8 @@ def main():
               f"GPUs, see the design notes)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy_mib:
+        calibrate_copy(args.calibrate_copy_mib)
     if args.workload == "batch":
         if args.cpu_chains == 32 and args.cpu_steps == 300:
             args.cpu_chains, args.cpu_steps = 16, 40
@@ -376,7 +423,12 @@ def main():
     from achr import stats as st
 
     if args.workload == "genome":
-        m, W = genome_model()
+        if args.load_inputs:
+            m, W = load_inputs(args.load_inputs)
+        else:
+            m, W = genome_model()
+            if args.save_inputs:
+                save_inputs(args.save_inputs, m, W)
         P, S = 20000, 5000
     else:
         m, W = ecoli_model()
@@ -418,18 +470,29 @@ def main():
         out.update({k: v for k, v in plain.items() if k != "kernel_ms"})
         ess_per_step_ref = plain_q["ess_per_step"]
     else:
-        # 1. training set: plain ACHR (residual BF16x3 projection) with every chain traced
-        t = time.time()
-        topts = achr.Options(lanes=args.lanes, trace_chains=P, trace_thin=args.train_thin,
-                             proj_prec=PROJ["res_bf16x3"], reproject_mode=2)
-        smp = achr.Sampler(model, W, topts)
-        smp.run(P, args.train_steps, seed=99)
-        smp.timing()
-        t_train_run = time.time() - t
-        # 2. rounding map + network + direction bank
-        t = time.time()
-        rep = smp.learn_directions(rank=args.rank, width=512, iters=args.iters, batch=2048, lr=1e-3, seed=7)
-        t_learn = time.time() - t
+        bank_file = os.path.join(args.load_inputs, "bank.npy") if args.load_inputs else None
+        if bank_file and os.path.exists(bank_file):
+            # a bank learned and saved by an earlier run (--save-inputs)
+            smp = achr.Sampler(model, W, achr.Options(lanes=args.lanes, reproject_mode=2))
+            smp.set_directions(np.load(bank_file))
+            smp.use_directions(True)
+            t_train_run = t_learn = 0.0
+            rep = {"bank": "loaded"}
+        else:
+            # 1. training set: plain ACHR (residual BF16x3 projection) with every chain traced
+            t = time.time()
+            topts = achr.Options(lanes=args.lanes, trace_chains=P, trace_thin=args.train_thin,
+                                 proj_prec=PROJ["res_bf16x3"], reproject_mode=2)
+            smp = achr.Sampler(model, W, topts)
+            smp.run(P, args.train_steps, seed=99)
+            smp.timing()
+            t_train_run = time.time() - t
+            # 2. rounding map + network + direction bank
+            t = time.time()
+            rep = smp.learn_directions(rank=args.rank, width=512, iters=args.iters, batch=2048, lr=1e-3, seed=7)
+            t_learn = time.time() - t
+            if args.save_inputs:
+                np.save(os.path.join(args.save_inputs, "bank.npy"), smp.directions())
         log(f"training run {t_train_run:.1f}s, learn {t_learn:.1f}s: {rep}")
         # 3. timed learned run
         smp.set_run_options(tc, args.trace_thin, args.check_every)

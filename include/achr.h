@@ -290,6 +290,10 @@ int achr_sampler_set_run_options(achr_sampler *sampler, int trace_chains, int tr
 
 /* Supplies the bank directly (n x K column major, host array). */
 int achr_sampler_set_directions(achr_sampler *sampler, const double *bank, int K);
+/* Copies the sampler's bank out (n x K column major) and its K; with bank
+ * NULL only K. A learned bank saved this way can be reloaded with
+ * achr_sampler_set_directions, skipping the training. */
+int achr_sampler_get_directions(const achr_sampler *sampler, double *bank, int *K);
 /* 1: bank directions; 0: ACHR directions (the reference's). */
 int achr_sampler_use_directions(achr_sampler *sampler, int on);
 
