@@ -259,6 +259,22 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
     }
   };
   load_dir(attempt);
+  // Bank mode clips the last accepted step lazily (at the next bounds loop);
+  // points that leave the kernel are clipped here first. clip_due is
+  // uniform over the chain: set by an accepted step (or a resume after the
+  // check of one), cleared by a clip.
+  bool clip_due = pending;
+  auto clip_all = [&]() {
+    if (!clip_due) return;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int off = (lane + k * TPC) * 8;
+      const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
+      if (x[k] > ubi) x[k] = ubi;
+      else if (x[k] < lbi) x[k] = lbi;
+    }
+    clip_due = false;
+  };
 
   for (;;) {
     if (!pending) {
@@ -308,8 +324,16 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
         const int off = (lane + k * TPC) * 8;
         const double ui = DIR ? u[k] : u[k] / nrm;
         u[k] = ui;
-        const double du = bload(r_ub, off) - x[k];
-        const double dl = x[k] - bload(r_lb, off);
+        const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
+        if constexpr (DIR) {
+          // the clip of the last accepted step, deferred to here where the
+          // bounds are loaded anyway (only after an accepted step: the
+          // starting point is never clipped)
+          const double xc = x[k] > ubi ? ubi : (x[k] < lbi ? lbi : x[k]);
+          x[k] = clip_due ? xc : x[k];
+        }
+        const double du = ubi - x[k];
+        const double dl = x[k] - lbi;
         // Both branches of the reference (ACHRCuda.cu:270-279) divide the same
         // two distances by u_i; divide once and select, so a wavefront whose
         // lanes disagree on the sign does not run both sets of divisions. A
@@ -342,6 +366,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
           if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
       }
+      clip_due = false;
       double* rmm = red + ((DIR && (attempt & 1u)) ? kRedMinMax1 : kRedMinMax0) * kRedSlots;
       group_minmax<TPC>(mn, mx, rmm, rmm + kRedSlots);
       const bool miss = !(nrm > 0.0) || !(mn > -INFINITY) || !(mx < INFINITY) ||
@@ -361,6 +386,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
       const double t = fma(rs, mx - mn, mn);
 #pragma unroll
       for (int k = 0; k < E; ++k) x[k] = fma(t, u[k], x[k]);
+      clip_due = true;
       // Prefetch the next attempt's point once this direction is consumed
       // (never two of them in registers); it lands during the residual
       // check, the clip, the centre update and the next step's set-up.
@@ -413,29 +439,31 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
       }
     }
     pending = false;
-    const double alpha = static_cast<double>(a.W + s) / static_cast<double>(a.W + s + 1);
-    const double beta = 1.0 / static_cast<double>(a.W + s + 1);
+    if constexpr (!DIR) {
+      // clip, then the centre update with the clipped point
+      const double alpha = static_cast<double>(a.W + s) / static_cast<double>(a.W + s + 1);
+      const double beta = 1.0 / static_cast<double>(a.W + s + 1);
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const int off = (lane + k * TPC) * 8;
-      const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
-      if (x[k] > ubi) x[k] = ubi;
-      else if (x[k] < lbi) x[k] = lbi;
-      if constexpr (DIR) {
-        // fixed direction law: no centre
-      } else if constexpr (kCL) {
-        cl[lane + k * TPC] = fma(alpha, cl[lane + k * TPC], beta * x[k]);
-      } else {
-        c[k] = fma(alpha, c[k], beta * x[k]);
-      }
-      if constexpr (E > 4) {
-        if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      for (int k = 0; k < E; ++k) {
+        const int off = (lane + k * TPC) * 8;
+        const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
+        if (x[k] > ubi) x[k] = ubi;
+        else if (x[k] < lbi) x[k] = lbi;
+        if constexpr (kCL) {
+          cl[lane + k * TPC] = fma(alpha, cl[lane + k * TPC], beta * x[k]);
+        } else {
+          c[k] = fma(alpha, c[k], beta * x[k]);
+        }
+        if constexpr (E > 4) {
+          if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
       }
     }
     ++s;
     if (a.trace != nullptr && b < a.trace_chains && s % a.trace_thin == 0) {
       const int kk = s / a.trace_thin - 1;
       if (kk < a.trace_len) {
+        if constexpr (DIR) clip_all();  // the recorded point is the clipped one
         const Row<TPC> tr(a.trace, static_cast<long long>(b) * a.trace_len + kk, n, n);
 #pragma unroll
         for (int k = 0; k < E; ++k) tr.st(lane + k * TPC, x[k]);
@@ -449,6 +477,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
       if constexpr (!DIR) crow.st(lane + k * TPC, kCL ? cl[lane + k * TPC] : c[k]);
     st = make_int4(static_cast<int>(attempt), s, kPendingCheck, consec);
   } else {
+    if constexpr (DIR) clip_all();
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int i = lane + k * TPC;

@@ -334,6 +334,72 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_precision(args):
+    """Reduced-precision projection on the genome model with 65,536
+    concurrent chains: the projection rounds of every 10th step in FP64 (the
+    oracle's arithmetic, bit for bit), or on the matrix cores in FP32 or
+    split-BF16x3 (residual form x - S+ (S x)) with an FP64 round every 100
+    steps. Same seed for all three, so the reduced-precision runs are
+    compared with the FP64 run chain by chain."""
+    import achr
+    import scipy.sparse as sp
+    P, S = args.chains or 65536, args.chain_steps or 1000
+    if args.load_inputs:
+        m, W = load_inputs(args.load_inputs)
+    else:
+        m, W = genome_model()
+    n, d = m.n, m.N.shape[1]
+    model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
+    model.set_nullspace(m.N)
+    Ssp = sp.csr_matrix((m.vals, (m.rows, m.cols)), shape=(m.m, m.n))
+    rng = m.ub - m.lb
+    live = rng > 0
+    out = {"n_gpus": 1, "steps": 1, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 bounds / f32, bf16x3 projection", "data": "synthetic"}
+    res, ref = {}, None
+    for name in ("fp64", "res_f32", "res_bf16x3"):
+        t = time.time()
+        smp = achr.Sampler(model, W, achr.Options(proj_prec=PROJ[name], fp64_every=args.fp64_every,
+                                                  reproject_mode=2))
+        log(f"{name}: sampler ready in {time.time() - t:.1f}s")
+        wall, tm = measure(smp, P, S, 1, args.warmup, seed0=7)
+        pts, st = smp.points(), smp.stats()
+        del smp
+        r = {"s_per_file": wall, "chain_steps_per_s": P * S / wall, "kernel_ms": tm,
+             "drift_inf": float(np.abs(Ssp @ pts).max()), "reprojections": int(st[:, 1].sum()),
+             "checks": int(st[:, 2].sum())}
+        if ref is None:
+            ref = pts
+        else:
+            A, B = pts[live], ref[live]
+            dm = np.abs(A.mean(1) - B.mean(1))
+            dsd = np.abs(A.std(1) - B.std(1))
+            r["vs_fp64"] = {"mean_frac_max": float((dm / rng[live]).max()),
+                            "std_frac_max": float((dsd / rng[live]).max()),
+                            "point_abs_diff_max_frac": float((np.abs(A - B).max(1) / rng[live]).max()),
+                            "note": "same seed: chain-by-chain against the FP64 run, fractions of ub - lb"}
+            del A, B
+        # projection arithmetic: FP64 rounds 4 n d flop per flagged chain;
+        # residual rounds 2 n m (the S+ contraction; S x comes from the check)
+        fp64_rounds = tm["proj_launches"] if name == "fp64" else tm["proj_launches"] // args.fp64_every
+        low_rounds = tm["proj_launches"] - fp64_rounds
+        per_round_chains = r["reprojections"] / max(tm["proj_launches"], 1)
+        r["proj_tflops"] = (per_round_chains * (fp64_rounds * 4.0 * n * d + low_rounds * 2.0 * n * m.m)
+                            / (tm["ms_proj"] * 1e-3) / 1e12)
+        res[name] = r
+        log(f"{name}: {r}")
+    base = res["fp64"]["s_per_file"]
+    for name in ("res_f32", "res_bf16x3"):
+        res[name]["speedup_vs_fp64"] = base / res[name]["s_per_file"]
+    best = res["res_bf16x3"]
+    out.update({"metric": "chain-steps/s (genome model, 65,536 chains, split-BF16x3 projection rounds)",
+                "value": best["chain_steps_per_s"], "unit": "chain-steps/s", "ms_per_step": best["s_per_file"] * 1e3,
+                "config": {"workload": f"precision: {m.m}x{n} model (d={d}), {P} chains x {S} steps, "
+                                       f"FP64 round every {args.fp64_every * 10} steps", "chains": P, "chain_steps": S},
+                "variants": res})
+    print(json.dumps(out), flush=True)
+
+
 _MODEL_FIELDS = ("rows", "cols", "vals", "lb", "ub", "x0", "N")
 
 
@@ -379,7 +445,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed files (bench steps)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed files (short: 20 chain steps)")
-    ap.add_argument("--workload", choices=["genome", "ecoli", "batch"], default="genome")
+    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision"], default="genome")
     ap.add_argument("--models", type=int, default=64, help="batch workload: number of models")
     ap.add_argument("--probe-steps", type=int, default=20, help="batch workload: autotuner probe length")
     ap.add_argument("--sampler", choices=["learned", "plain"], default="learned")
@@ -413,6 +479,9 @@ def main():
 
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
+    if args.workload == "precision":
+        run_precision(args)
+        return
     if args.workload == "batch":
         if args.cpu_chains == 32 and args.cpu_steps == 300:
             args.cpu_chains, args.cpu_steps = 16, 40

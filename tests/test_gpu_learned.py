@@ -96,13 +96,14 @@ def test_bank_mode_bitwise_against_oracle(c1):
     model = achr.Model.from_dense(S, lb, ub)
     model.set_nullspace(N)
     rp, ci, va = O.csr_of(S)
-    for lanes, mode in ((32, 1), (64, 2), (256, 1)):
+    for lanes, mode in ((32, 1), (64, 2), (256, 1), (1024, 2)):
         print(f"bank lanes={lanes} mode={mode}", flush=True)
-        smp = achr.Sampler(model, Wf, achr.Options(lanes=lanes, reproject_mode=mode))
+        smp = achr.Sampler(model, Wf, achr.Options(lanes=lanes, reproject_mode=mode, trace_chains=5, trace_thin=7))
         smp.set_directions(bank)
         smp.use_directions(True)
         g = smp.sample(96, 150, seed=4)
-        o, ost, _ = O.sample(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S.shape[0], P=96, S=150,
-                             seed=4, lanes=lanes, bank=bank)
+        o, ost, otr = O.sample(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S.shape[0], P=96,
+                               S=150, seed=4, lanes=lanes, bank=bank, trace_chains=5, trace_thin=7)
         np.testing.assert_array_equal(smp.stats(), ost)
         np.testing.assert_array_equal(g, o)
+        np.testing.assert_array_equal(smp.trace(), otr)  # recorded points are the clipped ones
