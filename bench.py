@@ -32,6 +32,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "achr.cu_amd"))
 
 PROJ = {"fp64": 0, "res_bf16x3": 1, "res_f32": 2, "null_f32": 3, "null_bf16x3": 4}
+# HBM bytes of one full segment launch of the learned step kernel, measured with
+# rocprofv3 FETCH_SIZE / WRITE_SIZE and calibrated on a 1 GiB device copy
+# (profiles/step_kernel_bank_genome_pmc_traffic.md), keyed by
+# (reactions, chains, steps per launch); other shapes report null.
+PMC_TRAFFIC = {(10600, 20000, 100): (180.3e9, 173.0e9, "profiles/step_kernel_bank_genome_pmc_traffic.md")}
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
@@ -265,12 +270,13 @@ def run_batch(args):
         c["models"] += 1
         c["n_range"] = [min(c["n_range"][0], info[k]["n"]), max(c["n_range"][1], info[k]["n"])]
     log(f"autotune {t_tune:.1f}s: {classes}")
+    graph = not args.no_graph
     for i in range(args.warmup):
-        batch.run([P] * count, min(S, 20), seed=1000 + i)
+        batch.run([P] * count, min(S, 20), seed=1000 + i, graph=graph)
     walls, tms = [], []
     for i in range(args.steps):
         t = time.perf_counter()
-        batch.run([P] * count, S, seed=seed + i, file=i)
+        batch.run([P] * count, S, seed=seed + i, file=i, graph=graph)
         walls.append(time.perf_counter() - t)
         tms.append(batch.timing())
         log(f"batch file {i}: {walls[-1]:.3f}s wall, {tms[-1]}")
@@ -314,7 +320,7 @@ def run_batch(args):
                                   f"(sum {sum_n}), m = 0.55 n, {P} chains x {S} steps each, 2n exact warmup points",
                       "models": count, "chains_per_model": P, "chain_steps": S, "reactions_sum": sum_n,
                       "reactions_min": int(min(ns)), "reactions_max": int(max(ns))},
-           "batch": {**tm, "autotune_s": t_tune, "build_s": t_build, "size_classes": classes},
+           "batch": {**tm, "graph": graph, "autotune_s": t_tune, "build_s": t_build, "size_classes": classes},
            "solo": {"s_total": wall_solo, "chain_steps_per_s": total_steps / wall_solo,
                     "batch_speedup": wall_solo / wall},
            "moment_agreement_vs_solo_fp64": {
@@ -448,6 +454,7 @@ def main():
     ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision"], default="genome")
     ap.add_argument("--models", type=int, default=64, help="batch workload: number of models")
     ap.add_argument("--probe-steps", type=int, default=20, help="batch workload: autotuner probe length")
+    ap.add_argument("--no-graph", action="store_true", help="batch workload: launch the segments eagerly")
     ap.add_argument("--sampler", choices=["learned", "plain"], default="learned")
     ap.add_argument("--chains", type=int, default=None)
     ap.add_argument("--chain-steps", type=int, default=None)
@@ -580,6 +587,12 @@ def main():
                     "learn": {"train_run_s": t_train_run, "learn_s": t_learn, **rep},
                     "kernel_ms": tm})
         out["roofline"] = roofline_for("fp64", m, n, d, P, S, q, tm, bank=True)
+        meas = PMC_TRAFFIC.get((n, P, args.check_every))
+        if meas and out["roofline"]["kernel"] == "step_kernel":
+            out["roofline"]["traffic"] = meas[0]
+            out["roofline"]["traffic_note"] = (f"HBM bytes of one {args.check_every}-step launch (FETCH_SIZE x 2.000 + "
+                                               f"WRITE_SIZE x 1.000, calibrated on a 1 GiB copy); algorithmic "
+                                               f"{meas[1]:.4g} B for that launch; {meas[2]}")
         if plain_q is not None:
             rng = m.ub - m.lb
             live = rng > 0
@@ -599,6 +612,10 @@ def main():
                 "mean_frac_median": float(np.median(dm)), "std_frac_median": float(np.median(ds)),
                 "mean_z_max": float((dm_abs / np.maximum(se_m, 1e-300)).max()),
                 "std_z_max": float((dsd_abs / np.maximum(se_s, 1e-300)).max()),
+                "mean_z_q50_q99": [float(np.quantile(dm_abs / np.maximum(se_m, 1e-300), q)) for q in (0.5, 0.99)],
+                "reactions_mean_frac_over_1pct": int((dm > 0.01).sum()),
+                "reactions_std_frac_over_1pct": int((ds > 0.01).sum()),
+                "live_reactions": int(live.sum()),
                 "z_max_of_pure_noise_approx": float(np.sqrt(2.0 * np.log(int(live.sum())))),
                 "note": "|learned - plain| over the live reactions, as a fraction of ub - lb and in standard errors "
                         "of the difference (P independent chain endpoints each)"}
