@@ -121,12 +121,22 @@ def measure(smp, P, S, steps, warmup, seed0=1):
     return times[j], timings[j]
 
 
+def moving(lb, ub, *point_sets):
+    """Reactions with a range that the chains actually move: ub > lb and a
+    sample spread above 1e-9 of the range in at least one point set.
+    Reactions pinned by S once the fixed reactions are fixed only carry
+    rounding noise, which would dominate standard-error and R-hat maxima."""
+    rng = ub - lb
+    spread = np.max([p.std(1) for p in point_sets], axis=0)
+    return (rng > 0) & (spread > 1e-9 * rng)
+
+
 def sample_quality(m, smp, S, trace_stride_chains, P):
     from achr import stats as st
     stats_ = smp.stats()
     pts = smp.points()
     trace = smp.trace()
-    live = (m.ub - m.lb) > 0
+    live = moving(m.lb, m.ub, pts)
     ess = st.bulk_ess(trace[:, :, live])
     min_ess = float(np.nanmin(ess))
     if m.S is not None:
@@ -298,7 +308,7 @@ def run_batch(args):
         sm = models[k][0]
         pb, ps = pts_b[k], s.points()
         rng_ = sm.ub - sm.lb
-        live = rng_ > 0
+        live = moving(sm.lb, sm.ub, pb, s.points())
         dm = np.abs(pb.mean(1) - ps.mean(1))[live]
         se = np.sqrt((pb.var(1) + ps.var(1))[live] / P)
         dmean.append(float((dm / rng_[live]).max()))
@@ -406,6 +416,75 @@ def run_precision(args):
     print(json.dumps(out), flush=True)
 
 
+def run_convergence(args):
+    """Moment error of plain and learned ACHR against a long plain run on the
+    genome model: the reference is plain ACHR with --ref-steps steps (residual
+    split-BF16x3 projection rounds with FP64 every 100 steps, within 3e-5 of
+    range of the FP64 rounds); each short run is scored per reaction, as a
+    fraction of ub - lb, against it. All runs start at the warmup centre."""
+    import achr
+    P = args.chains or 20000
+    S_short, S_long = args.chain_steps or 5000, args.ref_steps
+    if args.load_inputs:
+        m, W = load_inputs(args.load_inputs)
+    else:
+        m, W = genome_model()
+    n = m.n
+    model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
+    model.set_nullspace(m.N)
+    rng = m.ub - m.lb
+    live = rng > 0
+    tc = min(args.trace_chains, P)
+
+    def summary(pts, ref):
+        A, B = pts[live], ref[live]
+        dm = np.abs(A.mean(1) - B.mean(1)) / rng[live]
+        ds = np.abs(A.std(1) - B.std(1)) / rng[live]
+        return {"mean_frac_max": float(dm.max()), "std_frac_max": float(ds.max()),
+                "mean_frac_q99": float(np.quantile(dm, 0.99)), "std_frac_q99": float(np.quantile(ds, 0.99)),
+                "mean_frac_median": float(np.median(dm)), "std_frac_median": float(np.median(ds)),
+                "reactions_over_1pct": int(((dm > 0.01) | (ds > 0.01)).sum())}
+
+    def run(smp, S, seed, thin):
+        from achr import stats as st
+        smp.set_run_options(tc, thin, smp.opts.check_every)
+        t = time.perf_counter()
+        smp.run(P, S, seed=seed)
+        tm = smp.timing()
+        wall = time.perf_counter() - t
+        tr = smp.trace()
+        rh = st.rhat(tr[:, :, live])
+        return smp.points(), {"s": wall, "kernel_ms": tm["ms_total"], "max_rhat": float(np.nanmax(rh))}
+
+    res = {}
+    plain = achr.Sampler(model, W, achr.Options(proj_prec=PROJ["res_bf16x3"], reproject_mode=2))
+    ref, info = run(plain, S_long, 11, max(S_long // 100, 1))
+    res[f"plain_{S_long}_reference"] = info
+    log(f"reference: {info}")
+    pts, info = run(plain, S_short, 12, max(S_short // 100, 1))
+    res[f"plain_{S_short}"] = {**info, **summary(pts, ref)}
+    log(f"plain short: {res[f'plain_{S_short}']}")
+    del plain, pts
+    learned = achr.Sampler(model, W, achr.Options(lanes=0, trace_chains=P, trace_thin=args.train_thin,
+                                                  proj_prec=PROJ["res_bf16x3"], reproject_mode=2))
+    learned.run(P, args.train_steps, seed=99)
+    learned.timing()
+    rep = learned.learn_directions(rank=args.rank, width=512, iters=args.iters, batch=2048, lr=1e-3, seed=7)
+    learned.opts.check_every = args.check_every
+    for S in (S_short, S_long):
+        pts, info = run(learned, S, 13, max(S // 100, 1))
+        res[f"learned_{S}"] = {**info, **summary(pts, ref)}
+        log(f"learned {S}: {res[f'learned_{S}']}")
+    out = {"n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "metric": f"max per-reaction |mean| / |std| error vs a {S_long}-step plain run (fraction of ub - lb)",
+           "value": res[f"learned_{S_short}"]["std_frac_max"], "unit": "fraction of range",
+           "config": {"workload": f"convergence: genome {m.m}x{n}, {P} chains, short runs {S_short} steps, "
+                                  f"reference {S_long} steps of plain ACHR", "chains": P},
+           "learn": rep, "runs": res}
+    print(json.dumps(out), flush=True)
+
+
 _MODEL_FIELDS = ("rows", "cols", "vals", "lb", "ub", "x0", "N")
 
 
@@ -451,7 +530,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed files (bench steps)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed files (short: 20 chain steps)")
-    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision"], default="genome")
+    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision", "convergence"], default="genome")
+    ap.add_argument("--ref-steps", type=int, default=20000, help="convergence workload: steps of the reference run")
     ap.add_argument("--models", type=int, default=64, help="batch workload: number of models")
     ap.add_argument("--probe-steps", type=int, default=20, help="batch workload: autotuner probe length")
     ap.add_argument("--no-graph", action="store_true", help="batch workload: launch the segments eagerly")
@@ -488,6 +568,9 @@ def main():
         calibrate_copy(args.calibrate_copy_mib)
     if args.workload == "precision":
         run_precision(args)
+        return
+    if args.workload == "convergence":
+        run_convergence(args)
         return
     if args.workload == "batch":
         if args.cpu_chains == 32 and args.cpu_steps == 300:
@@ -595,7 +678,7 @@ def main():
                                                f"{meas[1]:.4g} B for that launch; {meas[2]}")
         if plain_q is not None:
             rng = m.ub - m.lb
-            live = rng > 0
+            live = moving(m.lb, m.ub, q["pts"], plain_q["pts"])
             A, B = q["pts"][live], plain_q["pts"][live]
             dm_abs = np.abs(A.mean(1) - B.mean(1))
             dm = dm_abs / rng[live]
@@ -615,8 +698,8 @@ def main():
                 "mean_z_q50_q99": [float(np.quantile(dm_abs / np.maximum(se_m, 1e-300), q)) for q in (0.5, 0.99)],
                 "reactions_mean_frac_over_1pct": int((dm > 0.01).sum()),
                 "reactions_std_frac_over_1pct": int((ds > 0.01).sum()),
-                "live_reactions": int(live.sum()),
                 "z_max_of_pure_noise_approx": float(np.sqrt(2.0 * np.log(int(live.sum())))),
+                "moving_reactions": int(live.sum()),
                 "note": "|learned - plain| over the live reactions, as a fraction of ub - lb and in standard errors "
                         "of the difference (P independent chain endpoints each)"}
         ess_per_step_ref = plain_q["ess_per_step"] if plain_q is not None else None

@@ -107,3 +107,41 @@ def test_bank_mode_bitwise_against_oracle(c1):
         np.testing.assert_array_equal(smp.stats(), ost)
         np.testing.assert_array_equal(g, o)
         np.testing.assert_array_equal(smp.trace(), otr)  # recorded points are the clipped ones
+
+
+def test_learned_and_isotropic_directions_agree_at_convergence():
+    """On a 165 x 300 model, long enough to mix (3e6 steps), the learned
+    rounding map and isotropic free-null-space directions (exact hit-and-run)
+    both reach split-R-hat < 1.1 and their 1,000 chain endpoints agree within
+    sampling error per reaction: the learned law changes the speed, not the
+    target. (Plain ACHR started from these chord warmups is still unmixed at
+    this length; its adaptive centre is not a Markov chain.)"""
+    import scipy.linalg
+    m = synth.make_model(165, 300, 51, irrev_frac=0.3, n_fixed=6, width=(5.0, 50.0))
+    N = scipy.linalg.null_space(m.dense())
+    W = synth.chord_warmup(N, m.x0, m.lb, m.ub, 600, seed=52)
+    model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
+    model.set_nullspace(N)
+    P, S = 1000, 3_000_000
+    out = {}
+    for name, kw in (("learned", dict(rank=64, width=256, iters=300, batch=1024, lr=2e-3, seed=3)),
+                     ("isotropic", dict(isotropic=True, seed=4))):
+        print(f"{name}: training run + bank", flush=True)
+        smp = achr.Sampler(model, W, achr.Options(model_id=2, trace_chains=P, trace_thin=50))
+        smp.run(P, 2000, seed=9)
+        smp.learn_directions(**kw)
+        smp.set_run_options(64, S // 100, 100)
+        print(f"{name}: {S} steps", flush=True)
+        smp.run(P, S, seed=6)
+        out[name] = (smp.points(), smp.trace())
+    A, B = out["learned"][0], out["isotropic"][0]
+    rng = m.ub - m.lb
+    live = (rng > 0) & (np.maximum(A.std(1), B.std(1)) > 1e-9 * rng)
+    for name in out:
+        rh = np.nanmax(st.rhat(out[name][1][:, :, live]))
+        print(f"{name}: max split-R-hat {rh:.3f}", flush=True)
+        assert rh < 1.1
+    z = np.abs(A.mean(1) - B.mean(1))[live] / np.sqrt((A.var(1) + B.var(1))[live] / P)
+    zs = np.abs(A.std(1) - B.std(1))[live] / np.sqrt((A.var(1) + B.var(1))[live] / (2 * P))
+    print(f"max |z| of means {z.max():.2f}, of stds {zs.max():.2f} over {live.sum()} reactions", flush=True)
+    assert z.max() < 5.0 and zs.max() < 5.0
