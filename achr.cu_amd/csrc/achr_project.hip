@@ -5,7 +5,7 @@
 // (reprojectPoint / reprojectPoint2, ACHRcu/ACHRCuda.cu:208-230), which at
 // genome scale reads the whole of N (n x d doubles) per chain per check.
 // Here the flagged chains of a round are gathered and both products run as
-// register-tiled FP64 GEMMs, so N is read once per 64 chains. Each output is
+// register-tiled FP64 GEMMs, so N is read once per 128 chains. Each output is
 // accumulated with fused multiply-adds in increasing k, starting from +0.0 —
 // the loop order of the reference's two kernels — so the result is bitwise
 // the sequential dot product the host oracle computes. (On gfx950 FP64 vector
@@ -19,14 +19,23 @@
 namespace achr {
 namespace {
 
-constexpr int BM = 64, BN = 64, BK = 16, TM = 4, TN = 4;
+constexpr int BM = 128, BN = 128, BK = 8, TM = 8, TN = 8;
 constexpr int kThreads = (BM / TM) * (BN / TN);  // 256
+constexpr int kLoads = (BK * BM) / kThreads;      // 4 elements of A and of B per thread and slice
 
 // C(r, q) = sum_k A(r, k) * B(k, q) for r < M, k < K, q < *count.
 // A(r, k) = A[r*sar + k*sak]; column q of B starts at B + colB(q)*ldb and is
 // contiguous in k; column q of C starts at Cm + colC(q)*ldc, contiguous in r.
 // colB(q) = idxB ? idxB[q] : q, likewise colC.
-__global__ __launch_bounds__(kThreads) void dgemm_seq_kernel(int M, int K, const double* __restrict__ A,
+//
+// 128 x 128 output tile per 256-thread workgroup; thread (ty, tx) owns rows
+// ty*8 .. ty*8+7 and columns tx + 16 j (j < 8), 64 accumulators. Per k the
+// thread reads 8 + 8 doubles from LDS for 64 FMAs (2 bytes per FMA, what LDS
+// sustains at the FP64 FMA rate). The k loop runs over slices of 8 with two
+// LDS stages: the next slice is loaded into registers while the current one
+// is consumed, and stored into the other stage, one barrier per slice.
+// Padding past M, K or the column count reads 0.0, which adds nothing.
+__global__ __launch_bounds__(kThreads, 2) void dgemm_seq_kernel(int M, int K, const double* __restrict__ A,
                                                               long sar, long sak, const double* __restrict__ B,
                                                               long ldb, const int* __restrict__ idxB,
                                                               double* __restrict__ Cm, long ldc,
@@ -36,70 +45,93 @@ __global__ __launch_bounds__(kThreads) void dgemm_seq_kernel(int M, int K, const
   const int q0 = blockIdx.y * BN;
   if (q0 >= ncols) return;
   const int r0 = blockIdx.x * BM;
-  __shared__ double As[BK][BM + 2];
-  __shared__ double Bs[BK][BN + 2];
+  __shared__ double As[2][BK][BM + 4];  // padding 4: the k-strided stores spread over the banks
+  __shared__ double Bs[2][BK][BN + 4];
   const int t = threadIdx.x;
   const int ty = t / (BN / TN), tx = t % (BN / TN);
+
+  // What this thread loads of each slice (4 elements of A, 4 of B), mapped so
+  // that each load instruction of a wavefront covers whole segments: operands
+  // contiguous in k take 8 lanes per row (64 bytes), A contiguous in rows
+  // takes 64 consecutive rows of one k.
+  const bool a_kcontig = (sak == 1);
+  int ar[kLoads], ak[kLoads];
+#pragma unroll
+  for (int e = 0; e < kLoads; ++e) {
+    if (a_kcontig) {
+      ar[e] = (t >> 3) + 32 * e;
+      ak[e] = t & 7;
+    } else {
+      ar[e] = t & 127;
+      ak[e] = (t >> 7) + 2 * e;
+    }
+  }
+  const int bk = t & 7;
+  // Running pointers to this thread's elements of the current slice (null
+  // past M or past the column count), advanced by one slice per step.
+  const double* pA[kLoads];
+  const double* pB[kLoads];
+#pragma unroll
+  for (int e = 0; e < kLoads; ++e) {
+    const int gr = r0 + ar[e];
+    pA[e] = gr < M ? A + static_cast<long>(gr) * sar + static_cast<long>(ak[e]) * sak : nullptr;
+    const int gq = q0 + (t >> 3) + 32 * e;
+    pB[e] = gq < ncols ? B + (idxB ? static_cast<long>(idxB[gq]) : static_cast<long>(gq)) * ldb + bk : nullptr;
+  }
+  const long stepA = static_cast<long>(BK) * sak;
+
+  double pa[kLoads], pb[kLoads];
+  auto load_slice = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < kLoads; ++e) {
+      pa[e] = (pA[e] != nullptr && k0 + ak[e] < K) ? *pA[e] : 0.0;
+      pb[e] = (pB[e] != nullptr && k0 + bk < K) ? *pB[e] : 0.0;
+      if (pA[e] != nullptr) pA[e] += stepA;
+      if (pB[e] != nullptr) pB[e] += BK;
+    }
+  };
+  auto store_slice = [&](int st) {
+#pragma unroll
+    for (int e = 0; e < kLoads; ++e) {
+      As[st][ak[e]][ar[e]] = pa[e];
+      Bs[st][bk][(t >> 3) + 32 * e] = pb[e];
+    }
+  };
+
   double acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = 0.0;
 
-  // Column base pointers of the B columns this thread loads.
-  const double* bcol[(BK * BN) / kThreads];
-  int bk[(BK * BN) / kThreads], bq[(BK * BN) / kThreads];
-#pragma unroll
-  for (int e = 0; e < (BK * BN) / kThreads; ++e) {
-    const int lin = t + e * kThreads;
-    const int q = lin / BK, k = lin % BK;
-    bq[e] = q;
-    bk[e] = k;
-    const int gq = q0 + q;
-    bcol[e] = gq < ncols ? B + (idxB ? static_cast<long>(idxB[gq]) : static_cast<long>(gq)) * ldb : nullptr;
-  }
-  const bool a_kcontig = (sak == 1);
-
+  load_slice(0);
+  store_slice(0);
+  __syncthreads();
+  int st = 0;
   for (int k0 = 0; k0 < K; k0 += BK) {
-#pragma unroll
-    for (int e = 0; e < (BK * BM) / kThreads; ++e) {
-      const int lin = t + e * kThreads;
-      int r, k;
-      if (a_kcontig) {
-        r = lin / BK;
-        k = lin % BK;
-      } else {
-        r = lin % BM;
-        k = lin / BM;
-      }
-      const int gr = r0 + r, gk = k0 + k;
-      As[k][r] = (gr < M && gk < K) ? A[static_cast<long>(gr) * sar + static_cast<long>(gk) * sak] : 0.0;
-    }
-#pragma unroll
-    for (int e = 0; e < (BK * BN) / kThreads; ++e) {
-      const int gk = k0 + bk[e];
-      Bs[bk[e]][bq[e]] = (bcol[e] != nullptr && gk < K) ? bcol[e][gk] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
+    const bool more = k0 + BK < K;
+    if (more) load_slice(k0 + BK);
+#pragma unroll 2
     for (int k = 0; k < BK; ++k) {
       double av[TM], bv[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) av[i] = As[k][ty * TM + i];
+      for (int i = 0; i < TM; ++i) av[i] = As[st][k][ty * TM + i];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bv[j] = Bs[k][tx * TN + j];
+      for (int j = 0; j < TN; ++j) bv[j] = Bs[st][k][tx + 16 * j];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
     }
+    if (more) store_slice(st ^ 1);
     __syncthreads();
+    st ^= 1;
   }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const int gq = q0 + tx * TN + j;
-    if (gq >= ncols) continue;
-    double* ccol = Cm + (idxC ? static_cast<long>(idxC[gq]) : static_cast<long>(gq)) * ldc;
+    const int q = q0 + tx + 16 * j;
+    if (q >= ncols) continue;
+    double* ccol = Cm + (idxC ? static_cast<long>(idxC[q]) : static_cast<long>(q)) * ldc;
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
       const int gr = r0 + ty * TM + i;
