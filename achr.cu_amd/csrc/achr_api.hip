@@ -732,6 +732,7 @@ int segments_for(const achr_sampler* S, int Ssteps) {
 int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0, hipStream_t st,
                 achr::StepArgs* args) {
   if (P <= 0 || Ssteps < 0 || chain0 < 0) return fail(ACHR_E_ARG, "bad P=%d S=%d chain0=%d", P, Ssteps, chain0);
+  S->have_run = false;  // results of an earlier run are gone once its buffers are resized or reset
   const int n = S->n;
   const size_t Pn = static_cast<size_t>(P) * n;
   const int tr_chains = std::min(S->opt.trace_chains, P);
