@@ -366,6 +366,14 @@ def gemm_f32(A, B, prec: int = 0, alpha: float = 1.0, C=None, beta: float = 0.0,
     return out
 
 
+def bank_reciprocal(u) -> np.ndarray:
+    """The device reciprocal of the direction-bank chord, elementwise."""
+    u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+    r = np.empty_like(u)
+    check(lib().achr_bank_reciprocal(_dp(u), _dp(r), u.size), "achr_bank_reciprocal")
+    return r
+
+
 def _fp(a):
     return a.ctypes.data_as(_ctypes.POINTER(_ctypes.c_float))
 

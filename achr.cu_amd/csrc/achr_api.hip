@@ -996,6 +996,20 @@ int achr_sample_timing(achr_sampler* S, double* ms_total, double* ms_step, doubl
   return ACHR_OK;
 }
 
+int achr_bank_reciprocal(const double* u, double* r, long count) {
+  if (!u || !r || count < 0) return fail(ACHR_E_ARG, "bad reciprocal arguments");
+  if (count == 0) return ACHR_OK;
+  const size_t nc = static_cast<size_t>(count);
+  DevBuf<double> du, dr;
+  HIP_TRY(du.upload(u, nc, nullptr));
+  HIP_TRY(dr.ensure(nc));
+  const hipError_t e = achr::launch_bank_reciprocal(du.p, dr.p, nc, nullptr);
+  if (e != hipSuccess) return fail(ACHR_E_DEVICE, "reciprocal launch: %s", hipGetErrorString(e));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(r, dr.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
 int achr_gemm_f32(int M, int N, int K, const float* A, long sa_m, long sa_k, const float* B, long sb_k, long sb_n,
                   float* C, long sc_m, long sc_n, float alpha, float beta, int prec) {
   if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return fail(ACHR_E_ARG, "bad GEMM arguments");

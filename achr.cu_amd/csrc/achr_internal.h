@@ -86,5 +86,7 @@ hipError_t launch_to_f64(const float* src, double* dst, size_t count, hipStream_
 hipError_t sum_squares(const float* x, size_t count, double* out);
 // Elementwise FP64 -> FP32 copy of `count` values.
 hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_t stream);
+// r[i] = the bank-mode chord's reciprocal of u[i] (device arrays).
+hipError_t launch_bank_reciprocal(const double* u, double* r, size_t count, hipStream_t stream);
 
 }  // namespace achr

@@ -40,6 +40,20 @@ constexpr int kPrefetchMaxE = 8;  // largest slot count that prefetches warmup p
 // without one (direction bank) the min/max pair alternates with the attempt
 // parity.
 constexpr int kRedSum = 0, kRedMinMax0 = 1, kRedMax = 3, kRedMinMax1 = 4, kRedArrays = 6;
+// A chain that has its workgroup to itself keeps the bounds of its first
+// kLdsBoundSlots slots in LDS, as (lb, ub) pairs that each thread writes and
+// reads for its own elements only (no barrier); further slots read theirs
+// from L2 each step (holding them in registers instead spills). 9 slots of
+// pairs fill 144 KiB at 1,024 lanes, and at fewer lanes as much per thread,
+// so a CU still holds 1,024 such threads. A plain chain with more than 8
+// slots uses that LDS for its centre instead.
+constexpr int kLdsBoundSlots = 9;
+
+template <int TPC, int E, int CPB, bool DIR>
+struct BoundPlan {
+  static constexpr bool resident = CPB == 1 && (DIR || E <= 8);
+  static constexpr int lds = resident ? (E < kLdsBoundSlots ? E : kLdsBoundSlots) : 0;
+};
 
 // Buffer-resource loads: 32-bit byte offsets (one VGPR per slot, shared by
 // every per-reaction array) and hardware range checking — an offset at or
@@ -52,7 +66,7 @@ constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, gfx9-family data format
 constexpr int kOob = 0x7FFFFFF8;        // an offset every resource here rejects
 
 __device__ __forceinline__ rsrc_t make_rsrc(const double* p, long long bytes) {
-  const int nrec = bytes > 0x7FFFFFF0LL ? 0x7FFFFFF0 : static_cast<int>(bytes);
+  const int nrec = bytes <= 0 ? 0 : (bytes > 0x7FFFFFF0LL ? 0x7FFFFFF0 : static_cast<int>(bytes));
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), static_cast<short>(0), nrec, kRsrcWord3);
 }
 
@@ -92,16 +106,30 @@ __device__ __forceinline__ double group_sum(double v, double* red) {
   return v;
 }
 
-// Maxima and minima of values that are never NaN (chord ends, +-infinity,
-// absolute residuals): fmax / fmin give the same bits as compare-and-assign
-// and are single instructions.
+// Maximum and minimum of two doubles that are never NaN (chord ends,
+// +-infinity, absolute residuals; never a signed zero either): the same bits
+// as compare-and-assign. Written as the instructions themselves: fmax / fmin
+// would first quiet possible signalling NaNs in every operand that comes from
+// a shuffle or from LDS, one more instruction per operand.
+__device__ __forceinline__ double vmax(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double vmin(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// Maxima and minima of values that are never NaN over the chain's lanes.
 template <int TPC>
 __device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red_mn, double* red_mx) {
   constexpr int G = TPC < 64 ? TPC : 64;
 #pragma unroll
   for (int o = G / 2; o >= 1; o >>= 1) {
-    mn = fmax(mn, __shfl_xor(mn, o));
-    mx = fmin(mx, __shfl_xor(mx, o));
+    mn = vmax(mn, __shfl_xor(mn, o));
+    mx = vmin(mx, __shfl_xor(mx, o));
   }
   if constexpr (TPC > 64) {
     constexpr int NW = TPC / 64;
@@ -112,8 +140,8 @@ __device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
-      mn = fmax(mn, red_mn[i]);
-      mx = fmin(mx, red_mx[i]);
+      mn = vmax(mn, red_mn[i]);
+      mx = vmin(mx, red_mx[i]);
     }
   }
 }
@@ -122,15 +150,45 @@ template <int TPC>
 __device__ __forceinline__ double group_max(double v, double* red) {
   constexpr int G = TPC < 64 ? TPC : 64;
 #pragma unroll
-  for (int o = G / 2; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  for (int o = G / 2; o >= 1; o >>= 1) v = vmax(v, __shfl_xor(v, o));
   if constexpr (TPC > 64) {
     constexpr int NW = TPC / 64;
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < NW; ++i) v = fmax(v, red[i]);
+    for (int i = 0; i < NW; ++i) v = vmax(v, red[i]);
   }
   return v;
+}
+
+// Lane 0's value on every lane. For a value that already is equal on all
+// lanes of the wavefront (a reduced norm, chord end or residual of a chain
+// that owns whole wavefronts) this changes nothing, but it tells the compiler
+// so: the chain's counters, its branches and the Philox streams derived from
+// them then live in scalar registers and run on the scalar unit.
+__device__ __forceinline__ double wave_uniform(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(u & 0xFFFFFFFFu)));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(u >> 32)));
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// 1/b rounded to nearest for |b| in [2^-30, 2^30] (the components of a
+// bank direction that the chord uses: |u_i| > uTol): the hardware estimate
+// refined by three Newton steps y <- y + y (1 - b y), each residual 1 - b y
+// exact through the FMA. Two steps leave y within one ulp of 1/b (one of its
+// two neighbours); from such a y the third step rounds to nearest
+// (P. Markstein, IA-64 and Elementary Functions, 2000, ch. 8). Zero or
+// denormal components, which the chord never uses, may give a non-finite
+// value. Checked against IEEE division by achr_bank_reciprocal's test.
+__device__ __forceinline__ double rcp_rn(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  return fma(y, e, y);
 }
 
 __device__ __forceinline__ void bstore(rsrc_t r, int off, double v) {
@@ -174,7 +232,11 @@ struct Row {
 // arguments come by value: a reference to the kernel's argument block keeps
 // the compiler from scalarising it and costs about 40 more spilled VGPRs.
 template <int TPC, int E, int CPB, bool DIR>
-__device__ __forceinline__ void run_chain(const StepArgs a, const int b, double* lds) {
+__device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, double* lds) {
+  // With 64 or more lanes the chain owns whole wavefronts: its index is
+  // wavefront-uniform, and so is everything it branches on.
+  constexpr bool kUni = TPC >= 64;
+  const int b = kUni ? __builtin_amdgcn_readfirstlane(b_in) : b_in;
   double* red = lds;  // 4 reduction slot arrays; only block-per-chain schedules use them
   const int g = threadIdx.x / TPC;
   const int lane = threadIdx.x % TPC;
@@ -188,6 +250,19 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
 
   const rsrc_t r_lb = make_rsrc(a.lb, 8LL * n);
   const rsrc_t r_ub = make_rsrc(a.ub, 8LL * n);
+  using BP = BoundPlan<TPC, E, CPB, DIR>;
+  double2* bl = reinterpret_cast<double2*>(lds + kRedArrays * kRedSlots);  // (lb, ub) of the LDS slots
+#pragma unroll
+  for (int k = 0; k < BP::lds; ++k) {
+    const int off = (lane + k * TPC) * 8;
+    bl[lane + k * TPC] = make_double2(bload(r_lb, off), bload(r_ub, off));
+  }
+  // (lb, ub) of slot k, from LDS or memory
+  auto bounds = [&](int k) -> double2 {
+    if (k < BP::lds) return bl[lane + k * TPC];
+    const int off = (lane + k * TPC) * 8;
+    return make_double2(bload(r_lb, off), bload(r_ub, off));
+  };
   const Row<TPC> xrow(a.X, b, n, n);   // spill of x for the residual / projection
   const Row<TPC> crow(a.C, b, n, n);
   const Row<TPC> trow(a.T, b, d, d);   // N^T x of the in-kernel projection
@@ -268,10 +343,9 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
     if (!clip_due) return;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      const int off = (lane + k * TPC) * 8;
-      const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
-      if (x[k] > ubi) x[k] = ubi;
-      else if (x[k] < lbi) x[k] = lbi;
+      const double2 bk = bounds(k);
+      if (x[k] > bk.y) x[k] = bk.y;
+      else if (x[k] < bk.x) x[k] = bk.x;
     }
     clip_due = false;
   };
@@ -316,21 +390,23 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
       double nrm = 1.0;
       if constexpr (!DIR) {
         ss = group_sum<TPC>(ss, red + kRedSum * kRedSlots);
+        if constexpr (kUni) ss = wave_uniform(ss);
         nrm = sqrt(ss);
       }
       double mn = -INFINITY, mx = INFINITY;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const int off = (lane + k * TPC) * 8;
         const double ui = DIR ? u[k] : u[k] / nrm;
         u[k] = ui;
-        const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
+        const double2 bk = bounds(k);
+        const double lbi = bk.x, ubi = bk.y;
         if constexpr (DIR) {
           // the clip of the last accepted step, deferred to here where the
-          // bounds are loaded anyway (only after an accepted step: the
-          // starting point is never clipped)
-          const double xc = x[k] > ubi ? ubi : (x[k] < lbi ? lbi : x[k]);
-          x[k] = clip_due ? xc : x[k];
+          // bounds are at hand (only after an accepted step: the starting
+          // point is never clipped)
+          const bool above = clip_due && x[k] > ubi;
+          const bool below = clip_due && x[k] < lbi;
+          x[k] = above ? ubi : (below ? lbi : x[k]);
         }
         const double du = ubi - x[k];
         const double dl = x[k] - lbi;
@@ -341,7 +417,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
         // u_i and two products instead of two divisions.
         double r_lo, r_hi;
         if constexpr (DIR) {
-          const double rcp = 1.0 / ui;
+          const double rcp = rcp_rn(ui);
           r_lo = -dl * rcp;
           r_hi = du * rcp;
         } else {
@@ -357,8 +433,8 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
         const bool use = du > a.dTol && dl > a.dTol && fabs(ui) > a.uTol;
         const double lo = fmin(r_lo, r_hi);
         const double hi = fmax(r_lo, r_hi);
-        mn = use ? fmax(mn, lo) : mn;
-        mx = use ? fmin(mx, hi) : mx;
+        mn = vmax(mn, use ? lo : -INFINITY);
+        mx = vmin(mx, use ? hi : INFINITY);
         // Schedule four slots at a time: their bound loads overlap, while
         // hoisting every slot's loads and FP64 divisions ahead would multiply
         // the live registers by E.
@@ -366,9 +442,14 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
           if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
       }
+
       clip_due = false;
       double* rmm = red + ((DIR && (attempt & 1u)) ? kRedMinMax1 : kRedMinMax0) * kRedSlots;
       group_minmax<TPC>(mn, mx, rmm, rmm + kRedSlots);
+      if constexpr (kUni) {
+        mn = wave_uniform(mn);
+        mx = wave_uniform(mx);
+      }
       const bool miss = !(nrm > 0.0) || !(mn > -INFINITY) || !(mx < INFINITY) ||
                         (fabs(mn) < a.maxMinTol && fabs(mx) < a.maxMinTol) || (mn > mx);
       if (miss) {
@@ -409,6 +490,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
           if (rrow) rrow[j] = static_cast<float>(acc);
         }
         rho = group_max<TPC>(rho, red + kRedMax * kRedSlots);
+        if constexpr (kUni) rho = wave_uniform(rho);
         const bool reproj = rho > a.reprojTol && d > 0;
         if (reproj) ++sts.y;
         if (a.mode == 2) {
@@ -445,10 +527,9 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b, double*
       const double beta = 1.0 / static_cast<double>(a.W + s + 1);
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const int off = (lane + k * TPC) * 8;
-        const double ubi = bload(r_ub, off), lbi = bload(r_lb, off);
-        if (x[k] > ubi) x[k] = ubi;
-        else if (x[k] < lbi) x[k] = lbi;
+        const double2 bk = bounds(k);
+        if (x[k] > bk.y) x[k] = bk.y;
+        else if (x[k] < bk.x) x[k] = bk.x;
         if constexpr (kCL) {
           cl[lane + k * TPC] = fma(alpha, cl[lane + k * TPC], beta * x[k]);
         } else {
@@ -528,6 +609,21 @@ const void* kernel_for() {
 
 template <int TPC, bool DIR, bool BATCH>
 const void* kernel_for_e(int e) {
+  if constexpr (TPC == 1024) {
+    switch (e) {
+      case 3: return kernel_for<TPC, 3, DIR, BATCH>();
+      case 5: return kernel_for<TPC, 5, DIR, BATCH>();
+      case 6: return kernel_for<TPC, 6, DIR, BATCH>();
+      case 7: return kernel_for<TPC, 7, DIR, BATCH>();
+      case 9: return kernel_for<TPC, 9, DIR, BATCH>();
+      case 10: return kernel_for<TPC, 10, DIR, BATCH>();
+      case 11: return kernel_for<TPC, 11, DIR, BATCH>();
+      case 13: return kernel_for<TPC, 13, DIR, BATCH>();
+      case 14: return kernel_for<TPC, 14, DIR, BATCH>();
+      case 15: return kernel_for<TPC, 15, DIR, BATCH>();
+      default: break;
+    }
+  }
   switch (e) {
     case 1: return kernel_for<TPC, 1, DIR, BATCH>();
     case 2: return kernel_for<TPC, 2, DIR, BATCH>();
@@ -564,27 +660,51 @@ hipError_t allow_lds(const void* fn, size_t lds) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
 }
 
-constexpr int kElems[] = {1, 2, 4, 8, 12, 16};
+// Slot counts with a kernel: every count up to 16 for 1,024-lane chains
+// (the schedule of every model above 2,048 reactions, which then pads at
+// most one partial slot row), 1, 2, 4, 8, 12 and 16 for smaller groups.
+bool elems_instantiated(int tpc, int e) {
+  if (tpc == 1024) return e >= 1 && e <= 16;
+  return e == 1 || e == 2 || e == 4 || e == 8 || e == 12 || e == 16;
+}
 
 }  // namespace
 
 bool schedule_supported(int tpc, int e) {
   bool tpc_ok = tpc == 16 || tpc == 32 || tpc == 64 || tpc == 128 || tpc == 256 || tpc == 512 || tpc == 1024;
-  bool e_ok = false;
-  for (int v : kElems) e_ok = e_ok || v == e;
-  return tpc_ok && e_ok;
+  return tpc_ok && elems_instantiated(tpc, e);
 }
 
 int elems_for(int n, int tpc) {
   const int need = (n + tpc - 1) / tpc;
-  for (int v : kElems)
-    if (v >= need) return v;
+  for (int v = need < 1 ? 1 : need; v <= 16; ++v)
+    if (elems_instantiated(tpc, v)) return v;
   return -1;
+}
+
+namespace {
+__global__ __launch_bounds__(256) void bank_reciprocal_kernel(const double* __restrict__ u, double* __restrict__ r,
+                                                             size_t count) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < count;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    r[i] = rcp_rn(u[i]);
+}
+}  // namespace
+
+hipError_t launch_bank_reciprocal(const double* u, double* r, size_t count, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((count + 255) / 256, 65536);
+  hipLaunchKernelGGL(bank_reciprocal_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, u, r, count);
+  return hipGetLastError();
 }
 
 size_t step_lds_bytes(bool dir, const Schedule& s) {
   size_t bytes = kRedArrays * kRedSlots * sizeof(double);
   if (s.e > 8 && !dir) bytes += static_cast<size_t>(s.cpb) * s.tpc * s.e * sizeof(double);  // centres in LDS
+  if (s.cpb == 1 && (dir || s.e <= 8)) {
+    // resident (lb, ub) pairs (BoundPlan)
+    bytes += static_cast<size_t>(std::min(s.e, kLdsBoundSlots)) * s.tpc * 2 * sizeof(double);
+  }
   return bytes;
 }
 

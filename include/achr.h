@@ -322,6 +322,11 @@ void achr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t o
 int achr_gemm_f32(int M, int N, int K, const float *A, long sa_m, long sa_k, const float *B, long sb_k, long sb_n,
                   float *C, long sc_m, long sc_n, float alpha, float beta, int prec);
 
+/* The reciprocal the direction-bank chord computes on the device, r[i] =
+ * 1/u[i], for `count` host values (copied to and from the device). It must
+ * equal IEEE division for every |u| in [2^-30, 2^30]; tests hold it to that. */
+int achr_bank_reciprocal(const double *u, double *r, long count);
+
 #ifdef __cplusplus
 }
 #endif
