@@ -251,6 +251,12 @@ class Sampler:
               "achr_sampler_set_run_options")
         self.opts.trace_chains, self.opts.trace_thin, self.opts.check_every = trace_chains, trace_thin, check_every
 
+    def start_point(self) -> np.ndarray:
+        """Where the chains of the next run start (the centre, or the bank-mode start)."""
+        out = np.empty(self.n)
+        check(lib().achr_sampler_start_point(self._h, _dp(out)), "achr_sampler_start_point")
+        return out
+
     def use_directions(self, on: bool) -> None:
         check(lib().achr_sampler_use_directions(self._h, 1 if on else 0), "achr_sampler_use_directions")
 

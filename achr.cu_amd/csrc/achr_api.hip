@@ -687,6 +687,13 @@ int achr_sampler_center(const achr_sampler* S, double* c0) {
   return ACHR_OK;
 }
 
+int achr_sampler_start_point(const achr_sampler* S, double* x0) {
+  if (!S || !x0) return fail(ACHR_E_ARG, "NULL argument");
+  const std::vector<double>& v = S->use_dirs ? S->s0 : S->c0;
+  std::copy(v.begin(), v.end(), x0);
+  return ACHR_OK;
+}
+
 int achr_sampler_schedule(const achr_sampler* S, int* lanes, int* mode) {
   if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
   if (lanes) *lanes = S->sched.tpc;
@@ -773,7 +780,7 @@ int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   a.dir_mode = S->use_dirs;
   a.lb = S->lb.p;
   a.ub = S->ub.p;
-  a.c0 = S->c0d.p;
+  a.c0 = S->use_dirs ? S->s0d.p : S->c0d.p;
   a.rowptr = S->rowptr.p;
   a.colidx = S->colidx.p;
   a.vals = S->vals.p;

@@ -94,6 +94,8 @@ struct achr_sampler {
   achr_host::DevBuf<double> dirs;     // n x K column major, unit columns
   int K = 0;
   int use_dirs = 0;
+  std::vector<double> s0;             // start of bank-mode chains (empty until computed)
+  achr_host::DevBuf<double> s0d;
   // run state
   int P = 0, S = 0;
   achr_host::DevBuf<double> X, C, T, out, trace;

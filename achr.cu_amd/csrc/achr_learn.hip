@@ -188,9 +188,115 @@ int achr_sampler_set_run_options(achr_sampler* S, int trace_chains, int trace_th
   return ACHR_OK;
 }
 
+// Start of bank-mode chains: the point of {x : S x = 0, x_i = lb_i for every
+// fixed reaction (lb_i = ub_i)} nearest the warmup centre c0. Bank directions
+// keep both conditions, so chains started there stay on that set up to
+// rounding and the residual check never has to reproject. (c0 is off it by
+// the warmup file's rounding, and a chain started at c0 would undo every
+// reprojection at its next clip of the fixed reactions.) Host FP64, with the
+// orthonormal null-space basis N: y = N N^T c0 is the nearest point of
+// null(S); the nearest one that also meets x_F = lb_F is y + N da with da the
+// least-norm solution of N_F da = lb_F - y_F (N_F: the fixed rows of N,
+// possibly dependent). A pivoted, reorthogonalised Gram-Schmidt pass over the
+// rows of N_F picks independent rows P and gives N_P = L Q (Q orthonormal, L
+// lower triangular), so da = Q^T L^{-1} (lb_P - y_P). Finally x_F = lb_F.
+static int compute_bank_start(achr_sampler* S) {
+  const achr_model* M = S->model;
+  const int n = S->n, d = S->d;
+  std::vector<double> x = S->c0;
+  if (d > 0 && M->N.size() == static_cast<size_t>(n) * d) {
+    const double* N = M->N.data();
+    auto col = [&](int j) { return N + static_cast<size_t>(j) * n; };
+    std::vector<double> t(d, 0.0);
+    for (int j = 0; j < d; ++j) {
+      const double* cj = col(j);
+      double acc = 0.0;
+      for (int i = 0; i < n; ++i) acc += cj[i] * S->c0[i];
+      t[j] = acc;
+    }
+    std::fill(x.begin(), x.end(), 0.0);
+    for (int j = 0; j < d; ++j) {
+      const double* cj = col(j);
+      for (int i = 0; i < n; ++i) x[i] += cj[i] * t[j];
+    }
+    std::vector<int> F;
+    for (int i = 0; i < n; ++i)
+      if (M->lb[i] == M->ub[i]) F.push_back(i);
+    const int f = static_cast<int>(F.size());
+    if (f > 0) {
+      auto dot = [d](const double* a, const double* b) {
+        double s = 0.0;
+        for (int j = 0; j < d; ++j) s += a[j] * b[j];
+        return s;
+      };
+      std::vector<double> NF(static_cast<size_t>(f) * d), R;  // f x d row major
+      for (int r = 0; r < f; ++r)
+        for (int j = 0; j < d; ++j) NF[static_cast<size_t>(r) * d + j] = col(j)[F[r]];
+      R = NF;
+      double scale = 0.0;
+      for (int r = 0; r < f; ++r) scale = std::max(scale, std::sqrt(dot(&NF[static_cast<size_t>(r) * d], &NF[static_cast<size_t>(r) * d])));
+      std::vector<int> piv, rest(f);
+      for (int r = 0; r < f; ++r) rest[r] = r;
+      std::vector<double> Q;  // rank x d row major
+      while (!rest.empty()) {
+        int best = -1;
+        double bn = 0.0;
+        for (int k = 0; k < static_cast<int>(rest.size()); ++k) {
+          const double* rr = &R[static_cast<size_t>(rest[k]) * d];
+          const double v = std::sqrt(dot(rr, rr));
+          if (v > bn) { bn = v; best = k; }
+        }
+        if (best < 0 || !(bn > 1e-10 * scale)) break;
+        const int j = rest[best];
+        rest.erase(rest.begin() + best);
+        std::vector<double> q(R.begin() + static_cast<size_t>(j) * d, R.begin() + static_cast<size_t>(j + 1) * d);
+        for (int k = 0; k < static_cast<int>(piv.size()); ++k) {  // second orthogonalisation
+          const double* qk = &Q[static_cast<size_t>(k) * d];
+          const double c = dot(qk, q.data());
+          for (int jj = 0; jj < d; ++jj) q[jj] -= c * qk[jj];
+        }
+        const double qn = std::sqrt(dot(q.data(), q.data()));
+        for (double& v : q) v /= qn;
+        for (int rr : rest) {
+          double* row = &R[static_cast<size_t>(rr) * d];
+          const double c = dot(row, q.data());
+          for (int jj = 0; jj < d; ++jj) row[jj] -= c * q[jj];
+        }
+        Q.insert(Q.end(), q.begin(), q.end());
+        piv.push_back(j);
+      }
+      const int rk = static_cast<int>(piv.size());
+      // L z = lb_P - y_P with L[a][b] = N_F[piv[a]] . Q[b] (b <= a)
+      std::vector<double> z(rk, 0.0);
+      for (int a = 0; a < rk; ++a) {
+        const double* row = &NF[static_cast<size_t>(piv[a]) * d];
+        double v = M->lb[F[piv[a]]] - x[F[piv[a]]];
+        for (int b = 0; b < a; ++b) v -= dot(row, &Q[static_cast<size_t>(b) * d]) * z[b];
+        z[a] = v / dot(row, &Q[static_cast<size_t>(a) * d]);
+      }
+      std::vector<double> da(d, 0.0);
+      for (int a = 0; a < rk; ++a)
+        for (int jj = 0; jj < d; ++jj) da[jj] += Q[static_cast<size_t>(a) * d + jj] * z[a];
+      for (int j = 0; j < d; ++j) {
+        const double* cj = col(j);
+        for (int i = 0; i < n; ++i) x[i] += cj[i] * da[j];
+      }
+      for (int k = 0; k < f; ++k) x[F[k]] = M->lb[F[k]];
+    }
+  }
+  S->s0 = x;
+  HIP_TRY(S->s0d.upload(S->s0.data(), static_cast<size_t>(n), nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  return ACHR_OK;
+}
+
 int achr_sampler_use_directions(achr_sampler* S, int on) {
   if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
   if (on && S->K == 0) return fail(ACHR_E_STATE, "no direction bank");
+  if (on && S->s0.empty()) {
+    const int rc = compute_bank_start(S);
+    if (rc != ACHR_OK) return rc;
+  }
   S->use_dirs = on ? 1 : 0;
   return ACHR_OK;
 }
@@ -552,7 +658,8 @@ int achr_sampler_learn_directions(achr_sampler* S, const achr_learn_options* opt
   HIP_TRY(achr::launch_normalize_cols(S->dirs.p, n, K, nullptr));
   HIP_TRY(hipDeviceSynchronize());
   S->K = K;
-  S->use_dirs = 1;
+  const int rc_start = achr_sampler_use_directions(S, 1);
+  if (rc_start != ACHR_OK) return rc_start;
   if (report) {
     report[0] = df;
     report[1] = static_cast<double>(Msamp);

@@ -632,7 +632,7 @@ def main():
         bank_file = os.path.join(args.load_inputs, "bank.npy") if args.load_inputs else None
         if bank_file and os.path.exists(bank_file):
             # a bank learned and saved by an earlier run (--save-inputs)
-            smp = achr.Sampler(model, W, achr.Options(lanes=args.lanes, reproject_mode=2))
+            smp = achr.Sampler(model, W, achr.Options(lanes=args.lanes, proj_prec=PROJ["res_bf16x3"], reproject_mode=2))
             smp.set_directions(np.load(bank_file))
             smp.use_directions(True)
             t_train_run = t_learn = 0.0

@@ -132,7 +132,8 @@ void achr_options_default(achr_options *opts);
 /* Sampler over a model with a null space and a warmup set (n x W column
  * major, each point contiguous). Computes the warmup centre on the host as
  * (1/W) * sum_k w_k (createCenterPt, ACHRCuda.cu:414-434) and uploads the
- * warmup set, bounds, S and N to device memory. opts may be NULL. */
+ * warmup set, bounds, S and N to device memory. opts may be NULL. The model
+ * must outlive the sampler (learning a bank and bank mode read it). */
 int achr_sampler_create(const achr_model *model, const double *warmup, int W, const achr_options *opts,
                         achr_sampler **out);
 
@@ -294,8 +295,15 @@ int achr_sampler_set_directions(achr_sampler *sampler, const double *bank, int K
  * NULL only K. A learned bank saved this way can be reloaded with
  * achr_sampler_set_directions, skipping the training. */
 int achr_sampler_get_directions(const achr_sampler *sampler, double *bank, int *K);
-/* 1: bank directions; 0: ACHR directions (the reference's). */
+/* 1: bank directions; 0: ACHR directions (the reference's). Bank-mode
+ * chains start from the point of {x : S x = 0, fixed reactions at their
+ * bound} nearest the warmup centre (computed on the host when the bank is
+ * first switched on), ACHR chains from the warmup centre itself. */
 int achr_sampler_use_directions(achr_sampler *sampler, int on);
+/* The point every chain of the next run starts from (n doubles): the warmup
+ * centre with ACHR directions, the bank-mode start with bank directions. No
+ * counterpart in the reference (its chains all start at the centre). */
+int achr_sampler_start_point(const achr_sampler *sampler, double *x0);
 
 /* -------------------------------------------------------------- file I/O */
 

@@ -47,11 +47,11 @@ def cases(c1):
     return out
 
 
-def oracle_of(c, *, P, S, seed, file, model_id, lanes, bank=None):
+def oracle_of(c, *, P, S, seed, file, model_id, lanes, bank=None, c0=None):
     say(f"oracle n={c['S'].shape[1]} P={P} S={S} lanes={lanes} model={model_id}")
     pts, st, _ = O.sample(warm=c["warm"], lb=c["lb"], ub=c["ub"], rowptr=c["rp"], colidx=c["ci"], vals=c["va"],
                           N=c["N"], m=c["S"].shape[0], P=P, S=S, seed=seed, file=file, model=model_id, lanes=lanes,
-                          threads=8, bank=bank)
+                          threads=8, bank=bank, c0=c0)
     return pts, st
 
 
@@ -137,6 +137,6 @@ def test_batch_with_direction_bank_and_rerun(cases):
         lanes = batch.lanes()
         for k, cc in enumerate(cases):
             o_pts, o_st = oracle_of(cc, P=P[k], S=50, seed=seed, file=0, model_id=k + 1, lanes=int(lanes[k]),
-                                    bank=bank if k == 3 else None)
+                                    bank=bank if k == 3 else None, c0=smps[k].start_point())
             np.testing.assert_array_equal(smps[k].stats(), o_st)
             np.testing.assert_array_equal(smps[k].points(), o_pts)

@@ -103,10 +103,52 @@ def test_bank_mode_bitwise_against_oracle(c1):
         smp.use_directions(True)
         g = smp.sample(96, 150, seed=4)
         o, ost, otr = O.sample(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S.shape[0], P=96,
-                               S=150, seed=4, lanes=lanes, bank=bank, trace_chains=5, trace_thin=7)
+                               S=150, seed=4, lanes=lanes, bank=bank, trace_chains=5, trace_thin=7,
+                               c0=smp.start_point())
         np.testing.assert_array_equal(smp.stats(), ost)
         np.testing.assert_array_equal(g, o)
         np.testing.assert_array_equal(smp.trace(), otr)  # recorded points are the clipped ones
+
+
+def test_bank_chains_start_on_the_flux_set_and_never_reproject():
+    """The centre of a "%f" warmup is off S x = 0 by the file's rounding. Bank
+    chains start from the nearest point with S x = 0 and every fixed reaction
+    at its bound (checked against a dense least-squares solution); bank
+    directions keep both, so the residual checks of a bank run never flag and
+    the points stay on that set. The fixed rows of N are linearly dependent
+    here (80 rows of rank 79), the case a plain conjugate-gradient solve of the
+    fixed-reaction correction gets wrong."""
+    import scipy.linalg
+    m = synth.make_model(330, 600, 43, irrev_frac=0.3, n_fixed=80, width=(5.0, 50.0))
+    S = m.dense()
+    N = scipy.linalg.null_space(S)
+    W = synth.round_like_csv(synth.chord_warmup(N, m.x0, m.lb, m.ub, 800, seed=44))
+    model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
+    model.set_nullspace(N)
+    fixed = m.lb == m.ub
+    assert fixed.sum() == 80 and np.linalg.matrix_rank(N[fixed], tol=1e-10) == 79
+    smp = achr.Sampler(model, W, achr.Options(reproject_mode=2))
+    c = smp.center()
+    assert np.abs(S @ c).max() > 1e-8
+    rng = np.random.default_rng(2)
+    free = synth.free_directions(N, m.lb, m.ub)
+    bank = free @ rng.standard_normal((free.shape[1], 500))
+    smp.set_directions(bank / np.linalg.norm(bank, axis=0))
+    np.testing.assert_array_equal(smp.start_point(), c)  # ACHR directions start at the centre
+    smp.use_directions(True)
+    x0 = smp.start_point()
+    A = np.vstack([S, np.eye(m.n)[fixed]])
+    b = np.concatenate([np.zeros(m.m), m.lb[fixed]])
+    want = c - np.linalg.lstsq(A, A @ c - b, rcond=None)[0]
+    assert np.abs(x0 - want).max() < 1e-9
+    assert np.abs(S @ x0).max() < 1e-11
+    np.testing.assert_array_equal(x0[fixed], m.lb[fixed])
+    smp.run(200, 300, seed=6)
+    stt = smp.stats()
+    assert stt[:, 2].min() >= 29 and stt[:, 1].sum() == 0  # 30 checks per chain, none reprojects
+    pts = smp.points()
+    assert np.abs(S @ pts).max() < 1e-10
+    np.testing.assert_array_equal(pts[fixed], np.repeat(m.lb[fixed][:, None], 200, axis=1))
 
 
 def test_learned_and_isotropic_directions_agree_at_convergence():

@@ -36,10 +36,10 @@ def genome_case(mets, rxns, seed):
     return m, W, model, Sd
 
 
-def oracle_run(m, W, Sd, *, bank, lanes, P, S, seed, trace, thin):
+def oracle_run(m, W, Sd, *, bank, lanes, P, S, seed, trace, thin, c0=None):
     return O.sample(warm=W, lb=m.lb, ub=m.ub, rowptr=Sd.indptr.astype(np.int32), colidx=Sd.indices.astype(np.int32),
                     vals=Sd.data, N=m.N, m=m.m, P=P, S=S, seed=seed, lanes=lanes, threads=8, bank=bank,
-                    trace_chains=trace, trace_thin=thin)
+                    trace_chains=trace, trace_thin=thin, c0=c0)
 
 
 @pytest.mark.parametrize("mets,rxns,slots", [(5200, 9500, 10), (5800, 10600, 11), (6600, 12000, 12)])
@@ -56,7 +56,8 @@ def test_bank_chain_with_resident_bounds_bitwise(mets, rxns, slots):
     assert smp.schedule() == (1024, 2)
     P, S, seed = 6, 30, 9
     smp.run(P, S, seed=seed)
-    o, ost, otr = oracle_run(m, W, Sd, bank=bank, lanes=1024, P=P, S=S, seed=seed, trace=3, thin=7)
+    o, ost, otr = oracle_run(m, W, Sd, bank=bank, lanes=1024, P=P, S=S, seed=seed, trace=3, thin=7,
+                             c0=smp.start_point())
     np.testing.assert_array_equal(smp.stats(), ost)
     np.testing.assert_array_equal(smp.points(), o)
     np.testing.assert_array_equal(smp.trace(), otr)
