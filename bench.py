@@ -36,7 +36,7 @@ PROJ = {"fp64": 0, "res_bf16x3": 1, "res_f32": 2, "null_f32": 3, "null_bf16x3": 
 # rocprofv3 FETCH_SIZE / WRITE_SIZE and calibrated on a 1 GiB device copy
 # (profiles/step_kernel_bank_genome_pmc_traffic.md), keyed by
 # (reactions, chains, steps per launch); other shapes report null.
-PMC_TRAFFIC = {(10600, 20000, 100): (180.3e9, 173.0e9, "profiles/step_kernel_bank_genome_pmc_traffic.md")}
+PMC_TRAFFIC = {(10600, 20000, 100): (178.1e9, 173.0e9, "profiles/step_kernel_bank_genome_pmc_traffic.md")}
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
@@ -632,7 +632,9 @@ def main():
         bank_file = os.path.join(args.load_inputs, "bank.npy") if args.load_inputs else None
         if bank_file and os.path.exists(bank_file):
             # a bank learned and saved by an earlier run (--save-inputs)
-            smp = achr.Sampler(model, W, achr.Options(lanes=args.lanes, proj_prec=PROJ["res_bf16x3"], reproject_mode=2))
+            # (FP64 projection rounds: no pseudo-inverse to build, so no rocSOLVER in the process; bank chains
+            # start on S x = 0 and never reproject, so the rounds' precision does not matter)
+            smp = achr.Sampler(model, W, achr.Options(lanes=args.lanes, reproject_mode=2))
             smp.set_directions(np.load(bank_file))
             smp.use_directions(True)
             t_train_run = t_learn = 0.0
