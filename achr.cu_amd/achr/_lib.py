@@ -85,6 +85,9 @@ SIGNATURES = {
     "achr_batch_sample": (_I, [_P, _PI, _I, C.c_uint64, C.c_uint32, _I]),
     "achr_batch_timing": (_I, [_P, _PD, _PI, _PI, _PI, _PI]),
     "achr_sampler_start_point": (_I, [_P, _PD]),
+    "achr_sample_continue": (_I, [_P, _I]),
+    "achr_sampler_save_state": (_I, [_P, C.c_char_p]),
+    "achr_sampler_load_state": (_I, [_P, C.c_char_p]),
     "achr_bank_reciprocal": (_I, [_PD, _PD, C.c_long]),
     "achr_gemm_f32": (_I, [_I, _I, _I, C.POINTER(C.c_float), C.c_long, C.c_long, C.POINTER(C.c_float), C.c_long,
                            C.c_long, C.POINTER(C.c_float), C.c_long, C.c_long, C.c_float, C.c_float, _I]),

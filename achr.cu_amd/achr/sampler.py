@@ -8,6 +8,8 @@ from __future__ import annotations
 
 import ctypes as C
 import ctypes as _ctypes
+import os
+import struct
 from dataclasses import dataclass
 
 import numpy as np
@@ -192,6 +194,21 @@ class Sampler:
         """Queues one file (P chains x S accepted steps); results stay on device."""
         check(lib().achr_sample(self._h, P, S, C.c_uint64(seed), C.c_uint32(file), chain0), "achr_sample")
         self.P, self.S = P, S
+
+    def continue_run(self, S_more: int) -> None:
+        """Runs the chains of the last run (or loaded state) S_more steps further."""
+        check(lib().achr_sample_continue(self._h, S_more), "achr_sample_continue")
+        self.S += S_more
+
+    def save_state(self, path: str) -> None:
+        check(lib().achr_sampler_save_state(self._h, os.fsencode(path)), "achr_sampler_save_state")
+
+    def load_state(self, path: str) -> None:
+        """Loads a saved chain state (the header's P and S become this sampler's run size)."""
+        check(lib().achr_sampler_load_state(self._h, os.fsencode(path)), "achr_sampler_load_state")
+        with open(path, "rb") as f:
+            head = f.read(20)
+        _n, self.P, self.S = struct.unpack("<3i", head[8:20])
 
     def points(self) -> np.ndarray:
         """n x P, reaction major (the reference's File<k> layout)."""

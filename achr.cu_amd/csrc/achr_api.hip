@@ -6,6 +6,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <climits>
 #include <cerrno>
 #include <cfloat>
 #include <cmath>
@@ -769,7 +770,18 @@ int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   HIP_TRY(hipMemsetAsync(S->flag.p, 0, sizeof(int) * P, st));
   S->ev_step.clear();
   S->ev_proj.clear();
+  S->seed = seed;
+  S->file = file;
+  S->chain0 = chain0;
+  S->trace_chains_run = S->trace_len > 0 ? tr_chains : 0;
+  fill_args(S, P, Ssteps, seed, file, chain0, args);
+  return ACHR_OK;
+}
 
+void fill_args(const achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0,
+               achr::StepArgs* args) {
+  const int n = S->n;
+  const int tr_chains = std::min(S->opt.trace_chains, P);
   achr::StepArgs& a = *args;
   a = achr::StepArgs{};
   a.n = n;
@@ -805,14 +817,13 @@ int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   a.st = S->st.p;
   a.stats = S->stats.p;
   a.flag = S->flag.p;
-  a.R32 = res_form ? S->R32.p : nullptr;
+  const int prec = S->opt.proj_prec;
+  a.R32 = (prec == ACHR_PROJ_RES_BF16X3 || prec == ACHR_PROJ_RES_F32) ? S->R32.p : nullptr;
   a.out = S->out.p;
   a.trace = S->trace_len > 0 ? S->trace.p : nullptr;
   a.trace_chains = tr_chains;
-  S->trace_chains_run = S->trace_len > 0 ? tr_chains : 0;
   a.trace_thin = S->opt.trace_thin;
   a.trace_len = S->trace_len;
-  return ACHR_OK;
 }
 
 int projection_kind(const achr_sampler* S, int seg) {
@@ -900,6 +911,42 @@ int enqueue_projection(achr_sampler* S, int kind, hipStream_t st) {
   return ACHR_OK;
 }
 
+int enqueue_segments(achr_sampler* S, const achr::StepArgs& a, int segments, int checks_done, hipStream_t st) {
+  size_t evn = 0;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventRecord(S->ev_begin, st));
+  for (int seg = 0; seg < segments; ++seg) {
+    int rc = event_at(S, evn, &e0);
+    if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
+    if (rc != ACHR_OK) return rc;
+    HIP_TRY(hipEventRecord(e0, st));
+    const hipError_t le = achr::launch_step(a, S->sched, st);
+    if (le != hipSuccess) return fail(ACHR_E_DEVICE, "step kernel launch: %s", hipGetErrorString(le));
+    HIP_TRY(hipEventRecord(e1, st));
+    S->ev_step.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
+    evn += 2;
+    const int kind = seg + 1 < segments ? projection_kind(S, checks_done + seg) : kProjNone;
+    if (kind != kProjNone) {
+      rc = event_at(S, evn, &e0);
+      if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
+      if (rc != ACHR_OK) return rc;
+      HIP_TRY(hipEventRecord(e0, st));
+      rc = enqueue_projection(S, kind, st);
+      if (rc != ACHR_OK) return rc;
+      if (kind == kProjFP64) ++S->rounds_fp64; else ++S->rounds_low;
+      HIP_TRY(hipEventRecord(e1, st));
+      S->ev_proj.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
+      evn += 2;
+    }
+  }
+  HIP_TRY(hipEventRecord(S->ev_end, st));
+  S->run_use_dirs = S->use_dirs;
+  S->run_check_every = S->opt.check_every;
+  S->run_trace_thin = S->opt.trace_thin;
+  S->run_trace_chains = S->opt.trace_chains;
+  return ACHR_OK;
+}
+
 }  // namespace achr_host
 
 extern "C" {
@@ -912,35 +959,148 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   achr::StepArgs a{};
   rc = achr_host::prepare_run(S, P, Ssteps, seed, file, chain0, st, &a);
   if (rc != ACHR_OK) return rc;
-  size_t evn = 0;
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventRecord(S->ev_begin, st));
-  const int segments = achr_host::segments_for(S, Ssteps);
-  for (int seg = 0; seg < segments; ++seg) {
-    rc = event_at(S, evn, &e0);
-    if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
-    if (rc != ACHR_OK) return rc;
-    HIP_TRY(hipEventRecord(e0, st));
-    const hipError_t le = achr::launch_step(a, S->sched, st);
-    if (le != hipSuccess) return fail(ACHR_E_DEVICE, "step kernel launch: %s", hipGetErrorString(le));
-    HIP_TRY(hipEventRecord(e1, st));
-    S->ev_step.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
-    evn += 2;
-    const int kind = seg + 1 < segments ? achr_host::projection_kind(S, seg) : achr_host::kProjNone;
-    if (kind != achr_host::kProjNone) {
-      rc = event_at(S, evn, &e0);
-      if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
-      if (rc != ACHR_OK) return rc;
-      HIP_TRY(hipEventRecord(e0, st));
-      rc = achr_host::enqueue_projection(S, kind, st);
-      if (rc != ACHR_OK) return rc;
-      if (kind == achr_host::kProjFP64) ++S->rounds_fp64; else ++S->rounds_low;
-      HIP_TRY(hipEventRecord(e1, st));
-      S->ev_proj.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
-      evn += 2;
-    }
+  rc = achr_host::enqueue_segments(S, a, achr_host::segments_for(S, Ssteps), 0, st);
+  if (rc != ACHR_OK) return rc;
+  S->have_run = true;
+  return ACHR_OK;
+}
+
+int achr_sample_continue(achr_sampler* S, int S_more) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  if (S_more < 0) return fail(ACHR_E_ARG, "bad S_more=%d", S_more);
+  if (!S->have_run) return fail(ACHR_E_STATE, "no run to continue");
+  if (S->run_use_dirs != S->use_dirs || S->run_check_every != S->opt.check_every ||
+      S->run_trace_thin != S->opt.trace_thin || S->run_trace_chains != S->opt.trace_chains)
+    return fail(ACHR_E_STATE, "directions, check period or trace settings changed since the run");
+  if (S_more == 0) return ACHR_OK;
+  if (S->S > INT_MAX - S_more) return fail(ACHR_E_ARG, "too many steps");
+  const int S0 = S->S, S1 = S0 + S_more;
+  int rc = achr_host::run_limits(S, S->sched, S->P, S1);
+  if (rc != ACHR_OK) return rc;
+  hipStream_t st = S->stream;
+  if (S->trace_len > 0) {
+    // more rows per traced chain, keeping the rows already recorded
+    const int old_len = S->trace_len, new_len = S1 / S->opt.trace_thin;
+    const size_t row = static_cast<size_t>(S->n) * sizeof(double);
+    DevBuf<double> grown;
+    HIP_TRY(grown.ensure(static_cast<size_t>(S->trace_chains_run) * new_len * S->n));
+    HIP_TRY(hipMemcpy2DAsync(grown.p, row * new_len, S->trace.p, row * old_len, row * old_len, S->trace_chains_run,
+                             hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    S->trace = std::move(grown);
+    S->trace_len = new_len;
   }
-  HIP_TRY(hipEventRecord(S->ev_end, st));
+  S->have_run = false;
+  S->ev_step.clear();
+  S->ev_proj.clear();
+  achr::StepArgs a{};
+  achr_host::fill_args(S, S->P, S1, S->seed, S->file, S->chain0, &a);
+  const int ce = S->opt.check_every;
+  const int done = (S0 + ce - 1) / ce, total = (S1 + ce - 1) / ce;
+  S->S = S1;
+  rc = achr_host::enqueue_segments(S, a, S->mode == 2 ? total - done + 1 : 1, done, st);
+  if (rc != ACHR_OK) return rc;
+  S->have_run = true;
+  return ACHR_OK;
+}
+
+namespace {
+// Checkpoint file: this header, then X (P x n doubles, chain major), C (the
+// same, plain ACHR only), st and stats (P x 4 int32 each), little endian.
+struct StateHeader {
+  char magic[8];  // "ACHRST01"
+  int32_t n, P, S, lanes, use_dirs, warm_count, check_every, mode, model_id, chain0;
+  uint32_t file, pad;
+  uint64_t seed;
+};
+constexpr char kStateMagic[8] = {'A', 'C', 'H', 'R', 'S', 'T', '0', '1'};
+}  // namespace
+
+int achr_sampler_save_state(const achr_sampler* S, const char* path) {
+  if (!S || !path) return fail(ACHR_E_ARG, "NULL argument");
+  if (!S->have_run) return fail(ACHR_E_STATE, "no run");
+  HIP_TRY(hipEventSynchronize(S->ev_end));
+  const size_t Pn = static_cast<size_t>(S->P) * S->n;
+  std::vector<double> X(Pn), C(S->use_dirs ? 0 : Pn);
+  std::vector<int32_t> st(4 * static_cast<size_t>(S->P)), stats(4 * static_cast<size_t>(S->P));
+  HIP_TRY(hipMemcpy(X.data(), S->X.p, sizeof(double) * Pn, hipMemcpyDeviceToHost));
+  if (!C.empty()) HIP_TRY(hipMemcpy(C.data(), S->C.p, sizeof(double) * Pn, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(st.data(), S->st.p, sizeof(int32_t) * st.size(), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(stats.data(), S->stats.p, sizeof(int32_t) * stats.size(), hipMemcpyDeviceToHost));
+  StateHeader h{};
+  std::memcpy(h.magic, kStateMagic, 8);
+  h.n = S->n;
+  h.P = S->P;
+  h.S = S->S;
+  h.lanes = S->sched.tpc;
+  h.use_dirs = S->use_dirs;
+  h.warm_count = S->use_dirs ? S->K : S->W;
+  h.check_every = S->opt.check_every;
+  h.mode = S->mode;
+  h.model_id = S->opt.model_id;
+  h.chain0 = S->chain0;
+  h.file = S->file;
+  h.seed = S->seed;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(ACHR_E_IO, "cannot write %s: %s", path, std::strerror(errno));
+  bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(X.data(), sizeof(double), Pn, f) == Pn &&
+            (C.empty() || std::fwrite(C.data(), sizeof(double), Pn, f) == Pn) &&
+            std::fwrite(st.data(), sizeof(int32_t), st.size(), f) == st.size() &&
+            std::fwrite(stats.data(), sizeof(int32_t), stats.size(), f) == stats.size();
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(ACHR_E_IO, "write error on %s", path);
+  return ACHR_OK;
+}
+
+int achr_sampler_load_state(achr_sampler* S, const char* path) {
+  if (!S || !path) return fail(ACHR_E_ARG, "NULL argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(ACHR_E_IO, "cannot read %s: %s", path, std::strerror(errno));
+  StateHeader h{};
+  if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kStateMagic, 8) != 0) {
+    std::fclose(f);
+    return fail(ACHR_E_FORMAT, "%s is not a sampler state file", path);
+  }
+  const int warm_count = S->use_dirs ? S->K : S->W;
+  if (h.n != S->n || h.lanes != S->sched.tpc || h.use_dirs != S->use_dirs || h.warm_count != warm_count ||
+      h.check_every != S->opt.check_every || h.mode != S->mode || h.model_id != S->opt.model_id || h.P <= 0 ||
+      h.S < 0 || h.chain0 < 0) {
+    std::fclose(f);
+    return fail(ACHR_E_STATE, "%s was saved by a sampler with other settings (n, lanes, directions, warmup or bank "
+                "size, check period, reprojection mode or model_id)", path);
+  }
+  const size_t Pn = static_cast<size_t>(h.P) * h.n;
+  std::vector<double> X(Pn), C(h.use_dirs ? 0 : Pn);
+  std::vector<int32_t> st(4 * static_cast<size_t>(h.P)), stats(4 * static_cast<size_t>(h.P));
+  const bool ok = std::fread(X.data(), sizeof(double), Pn, f) == Pn &&
+                  (C.empty() || std::fread(C.data(), sizeof(double), Pn, f) == Pn) &&
+                  std::fread(st.data(), sizeof(int32_t), st.size(), f) == st.size() &&
+                  std::fread(stats.data(), sizeof(int32_t), stats.size(), f) == stats.size();
+  std::fclose(f);
+  if (!ok) return fail(ACHR_E_FORMAT, "%s is truncated", path);
+  int rc = achr_host::run_limits(S, S->sched, h.P, h.S);
+  if (rc != ACHR_OK) return rc;
+  hipStream_t stream = S->stream;
+  achr::StepArgs a{};
+  rc = achr_host::prepare_run(S, h.P, h.S, h.seed, h.file, h.chain0, stream, &a);
+  if (rc != ACHR_OK) return rc;
+  S->trace_len = 0;  // traces are not part of the state
+  S->trace_chains_run = 0;
+  std::vector<double> out(Pn);
+  for (int b = 0; b < h.P; ++b)
+    for (int i = 0; i < h.n; ++i) out[static_cast<size_t>(i) * h.P + b] = X[static_cast<size_t>(b) * h.n + i];
+  HIP_TRY(S->X.upload(X.data(), Pn, stream));
+  if (!C.empty()) HIP_TRY(S->C.upload(C.data(), Pn, stream));
+  HIP_TRY(S->out.upload(out.data(), Pn, stream));
+  HIP_TRY(hipMemcpyAsync(S->st.p, st.data(), sizeof(int32_t) * st.size(), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(S->stats.p, stats.data(), sizeof(int32_t) * stats.size(), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipEventRecord(S->ev_begin, stream));
+  HIP_TRY(hipEventRecord(S->ev_end, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  S->run_use_dirs = S->use_dirs;
+  S->run_check_every = S->opt.check_every;
+  S->run_trace_thin = S->opt.trace_thin;
+  S->run_trace_chains = S->opt.trace_chains;
   S->have_run = true;
   return ACHR_OK;
 }

@@ -98,6 +98,10 @@ struct achr_sampler {
   achr_host::DevBuf<double> s0d;
   // run state
   int P = 0, S = 0;
+  uint64_t seed = 0;                  // of the last run (achr_sample_continue repeats them)
+  uint32_t file = 0;
+  int chain0 = 0;
+  int run_use_dirs = 0, run_check_every = 0, run_trace_thin = 0, run_trace_chains = 0;
   achr_host::DevBuf<double> X, C, T, out, trace;
   achr_host::DevBuf<int4> st, stats;
   achr_host::DevBuf<int> flag, idx, count;
@@ -122,6 +126,12 @@ int segments_for(const achr_sampler* S, int Ssteps);
 // st and fills the kernel arguments.
 int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0, hipStream_t st,
                 achr::StepArgs* args);
+// Kernel arguments of a run of Ssteps steps over the sampler's current buffers.
+void fill_args(const achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0,
+               achr::StepArgs* args);
+// Queues the segments [0, segments) of a run (step launch, then the projection
+// round of checks already done + segment index), recording their events.
+int enqueue_segments(achr_sampler* S, const achr::StepArgs& a, int segments, int checks_done, hipStream_t st);
 // Projection round after segment seg (kProjNone outside mode 2).
 int projection_kind(const achr_sampler* S, int seg);
 // Queues one projection round over the chains flagged by the last segment.

@@ -243,8 +243,14 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
   if (b >= a.P) return;
   if (a.mode == 2 && lane == 0) a.flag[b] = 0;
   int4 st = a.st[b];
-  if (st.z == kDone) return;
   int4 sts = a.stats[b];
+  if (st.z == kDone) {
+    // A finished chain resumes where it stopped when its run is extended
+    // past its step count (its state is in X and C), unless the miss cap
+    // ended it.
+    if (st.y >= a.S || (sts.w & 1)) return;
+    st.z = kRunning;
+  }
   const int n = a.n;
   const int d = a.d;
 
@@ -563,6 +569,9 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
     for (int k = 0; k < E; ++k) {
       const int i = lane + k * TPC;
       if (i < n) a.out[static_cast<size_t>(i) * a.P + b] = x[k];
+      // the final state too, from which an extended run continues
+      xrow.st(i, x[k]);
+      if constexpr (!DIR) crow.st(i, kCL ? cl[i] : c[k]);
     }
     st = make_int4(static_cast<int>(attempt), s, kDone, consec);
   }

@@ -97,11 +97,61 @@ def cpu_baseline(m, W, N, P_cpu, S_cpu, threads, gpu_ess_per_step):
              S=S_cpu, seed=1, lanes=1024 if m.n > 4096 else 32, threads=threads)
     dt = time.time() - t
     steps_per_s = P_cpu * S_cpu / dt
+    # one core: an eighth of the chains (at least 2), same steps
+    P1 = max(2, P_cpu // 8)
+    t = time.time()
+    O.sample(warm=W, lb=m.lb, ub=m.ub, rowptr=rp, colidx=cols.astype(np.int32), vals=vals, N=N, m=m.m, P=P1,
+             S=S_cpu, seed=1, lanes=1024 if m.n > 4096 else 32, threads=1)
+    dt1 = time.time() - t
     return {"value": gpu_ess_per_step * steps_per_s, "unit": "min-ESS/s", "cores": threads, "kind": "port",
             "sample": f"oracle (C, OpenMP) {P_cpu} chains x {S_cpu} steps of the same model and warmup in {dt:.1f}s "
                       f"= {steps_per_s:.4g} chain-steps/s; min-ESS/s = that rate x the GPU run's min-ESS per "
                       f"chain-step (both run the identical, bit-equal algorithm)",
-            "chain_steps_per_s": steps_per_s}
+            "chain_steps_per_s": steps_per_s,
+            "single_core": {"chain_steps_per_s": P1 * S_cpu / dt1, "value": gpu_ess_per_step * P1 * S_cpu / dt1,
+                            "sample": f"{P1} chains x {S_cpu} steps in {dt1:.1f}s on 1 thread"},
+            "host": host_cpu()}
+
+
+def host_cpu():
+    """The host's CPU model and the cores this process may use."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": model, "usable_cores": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+
+
+def measure_copy_gbs(mib=1024, reps=5):
+    """Device-to-device copy rate (read + write bytes / time, best of `reps`):
+    the HBM rate a plain streaming kernel reaches on this GPU."""
+    import ctypes as C
+    import achr
+    achr.lib()
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+    best = float("inf")
+    try:
+        if hip.hipMemset(a, 1, nbytes) or hip.hipDeviceSynchronize():
+            raise RuntimeError("hipMemset failed")
+        for _ in range(reps):
+            t = time.perf_counter()
+            if hip.hipMemcpy(b, a, nbytes, 3) or hip.hipDeviceSynchronize():
+                raise RuntimeError("hipMemcpy failed")
+            best = min(best, time.perf_counter() - t)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * (mib << 20) / best / 1e9
 
 
 def measure(smp, P, S, steps, warmup, seed0=1):
@@ -672,6 +722,7 @@ def main():
                     "learn": {"train_run_s": t_train_run, "learn_s": t_learn, **rep},
                     "kernel_ms": tm})
         out["roofline"] = roofline_for("fp64", m, n, d, P, S, q, tm, bank=True)
+        out["roofline"]["measured_copy_GBs"] = measure_copy_gbs()
         meas = PMC_TRAFFIC.get((n, P, args.check_every))
         if meas and out["roofline"]["kernel"] == "step_kernel":
             out["roofline"]["traffic"] = meas[0]
