@@ -58,6 +58,8 @@ SIGNATURES = {
     "achr_sample_timing": (_I, [_P, _PD, _PD, _PD, _PI, _PI]),
     "achr_sampler_schedule": (_I, [_P, _PI, _PI]),
     "achr_write_points_csv": (_I, [C.c_char_p, _PD, _I, _I]),
+    "achr_write_points_bin": (_I, [C.c_char_p, _PD, _I, _I]),
+    "achr_read_points_bin": (_I, [C.c_char_p, C.POINTER(_PD), _PI, _PI]),
     "achr_read_warmup_csv": (_I, [C.c_char_p, _I, C.POINTER(_PD), _PI]),
     "achr_free": (None, [_P]),
     "achr_philox4x32_10": (None, [_PU32, _PU32, _PU32]),

@@ -347,6 +347,22 @@ def write_points_csv(path: str, points: np.ndarray) -> None:
     check(lib().achr_write_points_csv(str(path).encode(), _dp(p), p.shape[0], p.shape[1]), "write_points_csv")
 
 
+def write_points_bin(path: str, points: np.ndarray) -> None:
+    """Lossless point file: b"ACHRPT01", int32 n, int32 P, n x P float64 (reaction major)."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    check(lib().achr_write_points_bin(str(path).encode(), _dp(p), p.shape[0], p.shape[1]), "write_points_bin")
+
+
+def read_points_bin(path: str) -> np.ndarray:
+    ptr = C.POINTER(C.c_double)()
+    n, P = C.c_int(), C.c_int()
+    check(lib().achr_read_points_bin(str(path).encode(), C.byref(ptr), C.byref(n), C.byref(P)), "read_points_bin")
+    try:
+        return np.ctypeslib.as_array(ptr, shape=(n.value * P.value,)).reshape(n.value, P.value).copy()
+    finally:
+        lib().achr_free(ptr)
+
+
 def read_warmup_csv(path: str, n_expected: int = 0) -> np.ndarray:
     ptr = C.POINTER(C.c_double)()
     W = C.c_int()

@@ -1224,4 +1224,51 @@ int achr_write_points_csv(const char* path, const double* points, int n, int P) 
   return ACHR_OK;
 }
 
+namespace {
+constexpr char kPointsMagic[8] = {'A', 'C', 'H', 'R', 'P', 'T', '0', '1'};
+}
+
+int achr_write_points_bin(const char* path, const double* points, int n, int P) {
+  if (!path || !points || n <= 0 || P <= 0) return fail(ACHR_E_ARG, "bad arguments");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(ACHR_E_IO, "cannot open %s for writing", path);
+  const int32_t dims[2] = {n, P};
+  const size_t cnt = static_cast<size_t>(n) * P;
+  bool ok = std::fwrite(kPointsMagic, 1, 8, f) == 8 && std::fwrite(dims, sizeof(int32_t), 2, f) == 2 &&
+            std::fwrite(points, sizeof(double), cnt, f) == cnt;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(ACHR_E_IO, "write error on %s", path);
+  return ACHR_OK;
+}
+
+int achr_read_points_bin(const char* path, double** points, int* n, int* P) {
+  if (!path || !points || !n || !P) return fail(ACHR_E_ARG, "NULL argument");
+  *points = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(ACHR_E_IO, "cannot read %s: %s", path, std::strerror(errno));
+  char magic[8];
+  int32_t dims[2];
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kPointsMagic, 8) != 0 ||
+      std::fread(dims, sizeof(int32_t), 2, f) != 2 || dims[0] <= 0 || dims[1] <= 0) {
+    std::fclose(f);
+    return fail(ACHR_E_FORMAT, "%s is not a point file", path);
+  }
+  const size_t cnt = static_cast<size_t>(dims[0]) * dims[1];
+  double* buf = static_cast<double*>(std::malloc(sizeof(double) * cnt));
+  if (!buf) {
+    std::fclose(f);
+    return fail(ACHR_E_NOMEM, "out of host memory");
+  }
+  const bool ok = std::fread(buf, sizeof(double), cnt, f) == cnt;
+  std::fclose(f);
+  if (!ok) {
+    std::free(buf);
+    return fail(ACHR_E_FORMAT, "%s is truncated", path);
+  }
+  *points = buf;
+  *n = dims[0];
+  *P = dims[1];
+  return ACHR_OK;
+}
+
 }  // extern "C"

@@ -32,8 +32,12 @@ static int die(const char* what) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 6) {
-    std::fprintf(stderr, "Usage: %s <model.mps> <warmup.csv> <nFiles> <pointsPerFile> <stepsPerPoint>\n", argv[0]);
+  // The reference's five arguments; an optional sixth, --lossless, also
+  // writes each file's points bit-exact to File<k>.bin.
+  const bool lossless = argc == 7 && std::strcmp(argv[6], "--lossless") == 0;
+  if (argc != 6 && !lossless) {
+    std::fprintf(stderr, "Usage: %s <model.mps> <warmup.csv> <nFiles> <pointsPerFile> <stepsPerPoint> [--lossless]\n",
+                 argv[0]);
     return 2;
   }
   const auto t0 = std::chrono::steady_clock::now();
@@ -95,6 +99,8 @@ int main(int argc, char** argv) {
     const std::string name = "File" + std::to_string(k);
     if (achr_write_points_csv(name.c_str(), pts.data(), n, static_cast<int>(P)) != ACHR_OK)
       return die("writing points");
+    if (lossless && achr_write_points_bin((name + ".bin").c_str(), pts.data(), n, static_cast<int>(P)) != ACHR_OK)
+      return die("writing lossless points");
   }
   const double t_all = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::printf("Sampling done in %.5f seconds.\n", t_all);

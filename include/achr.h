@@ -329,6 +329,13 @@ int achr_sampler_start_point(const achr_sampler *sampler, double *x0);
  * (ACHRCuda.cu:744-756). */
 int achr_write_points_csv(const char *path, const double *points, int n, int P);
 
+/* Lossless counterpart of achr_write_points_csv (whose "%f" keeps six
+ * decimals): the 8 bytes "ACHRPT01", int32 n, int32 P, then the n x P
+ * doubles, reaction major, little endian. No counterpart in the reference. */
+int achr_write_points_bin(const char *path, const double *points, int n, int P);
+/* Reads such a file into a newly malloc'ed n x P array (release with achr_free). */
+int achr_read_points_bin(const char *path, double **points, int *n, int *P);
+
 /* Reads a warmup CSV into a newly malloc'ed n x W column-major array
  * (release with achr_free). */
 int achr_read_warmup_csv(const char *path, int n_expected, double **warmup, int *W);

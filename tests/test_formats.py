@@ -150,3 +150,23 @@ def test_options_model_id_default_and_range():
     o = _lib.AchrOptions()
     achr.lib().achr_options_default(ctypes.byref(o))
     assert o.model_id == 0 and o.fp64_every == 10
+
+
+def test_points_bin_round_trip_is_lossless(tmp_path):
+    """The lossless point file keeps every bit (signed zeros, subnormals,
+    infinities, NaN payloads) and its layout is the documented one."""
+    rng = np.random.default_rng(4)
+    pts = rng.standard_normal((7, 5)) * 10.0 ** rng.integers(-300, 300, (7, 5))
+    pts[0, :4] = [-0.0, 5e-324, np.inf, -np.inf]
+    pts[1, 0] = np.frombuffer(np.uint64(0x7FF8000000000123).tobytes(), dtype=np.float64)[0]
+    achr.write_points_bin(tmp_path / "p.bin", pts)
+    back = achr.read_points_bin(tmp_path / "p.bin")
+    np.testing.assert_array_equal(back.view(np.uint64), pts.view(np.uint64))
+    raw = (tmp_path / "p.bin").read_bytes()
+    assert raw[:8] == b"ACHRPT01" and np.frombuffer(raw[8:16], "<i4").tolist() == [7, 5]
+    np.testing.assert_array_equal(np.frombuffer(raw[16:], "<f8").reshape(7, 5).view(np.uint64), pts.view(np.uint64))
+    (tmp_path / "short.bin").write_bytes(raw[:-8])
+    with pytest.raises(achr.AchrError):
+        achr.read_points_bin(tmp_path / "short.bin")
+    with pytest.raises(achr.AchrError):
+        achr.read_points_bin(tmp_path / "missing.bin")

@@ -144,12 +144,13 @@ def test_miss_cap_forced_from_host():
 
 def test_cli_drop_in(tmp_path, c1_setup):
     """ACHRCuda model.mps warmup.csv nFiles nPoints nSteps writes File0..: n
-    lines x P comma-separated %f values, equal to the oracle's points."""
+    lines x P comma-separated %f values, equal to the oracle's points; with
+    --lossless also File<k>.bin, bit-equal to them."""
     npath = tmp_path / "N.bin"
     np.asfortranarray(c1_setup["N"]).ravel(order="F").tofile(npath)
     env = dict(os.environ, ACHR_SEED="77", ACHR_NULLSPACE=str(npath))
     say("cli run")
-    r = subprocess.run([achr.CLI_PATH, golden("c1.mps"), golden("c1_warmup.csv"), "2", "50", "80"],
+    r = subprocess.run([achr.CLI_PATH, golden("c1.mps"), golden("c1_warmup.csv"), "2", "50", "80", "--lossless"],
                        cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     for f in range(2):
@@ -160,6 +161,10 @@ def test_cli_drop_in(tmp_path, c1_setup):
                            file=f, lanes=32)
         expect = [",".join("%f" % v for v in o[i]) for i in range(95)]
         assert lines == expect
+        np.testing.assert_array_equal(achr.read_points_bin(tmp_path / f"File{f}.bin"), o)
+    r = subprocess.run([achr.CLI_PATH, golden("c1.mps"), golden("c1_warmup.csv"), "1", "5", "5", "--lossy"],
+                       cwd=tmp_path, env=env, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Usage" in r.stderr
     r = subprocess.run([achr.CLI_PATH, "missing.mps", "w.csv", "1", "1", "1"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0 and "MPS" in r.stderr
