@@ -11,6 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libachr.so")
 CLI_PATH = os.path.join(os.path.dirname(_HERE), "bin", "ACHRCuda")
+WARMUP_CLI_PATH = os.path.join(os.path.dirname(_HERE), "bin", "createWarmupPts")
 
 ACHR_OK = 0
 ERROR_NAMES = {
@@ -73,6 +74,9 @@ SIGNATURES = {
     "achr_dirnet_train": (_I, [_P, _I, _I, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
     "achr_dirnet_sample": (_I, [_P, C.POINTER(C.c_float), _I, _I, C.POINTER(C.c_float)]),
     "achr_learn_options_default": (None, [_P]),
+    "achr_warmup_options_default": (None, [_P]),
+    "achr_warmup_generate": (_I, [_P, _I, _P, _PD, _PD, _PI, _PD]),
+    "achr_write_fva_csv": (_I, [C.c_char_p, _P, _PD]),
     "achr_sampler_learn_directions": (_I, [_P, _P, _PD]),
     "achr_sampler_set_directions": (_I, [_P, _PD, _I]),
     "achr_sampler_use_directions": (_I, [_P, _I]),
@@ -113,6 +117,17 @@ class AchrOptions(C.Structure):
         ("proj_prec", C.c_int),
         ("fp64_every", C.c_int),
         ("model_id", C.c_int),
+    ]
+
+
+class AchrWarmupOptions(C.Structure):
+    """Mirror of ``achr_warmup_options`` (include/achr.h)."""
+
+    _fields_ = [
+        ("max_iters", C.c_int),
+        ("check_every", C.c_int),
+        ("tol", C.c_double),
+        ("seed", C.c_uint64),
     ]
 
 

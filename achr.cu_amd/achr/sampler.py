@@ -14,7 +14,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import AchrLearnOptions, AchrOptions, check, lib
+from ._lib import AchrLearnOptions, AchrOptions, AchrWarmupOptions, check, lib
 
 
 def _dp(a: np.ndarray):
@@ -345,6 +345,33 @@ def philox4x32_10(ctr, key) -> tuple:
 def write_points_csv(path: str, points: np.ndarray) -> None:
     p = np.ascontiguousarray(points, dtype=np.float64)
     check(lib().achr_write_points_csv(str(path).encode(), _dp(p), p.shape[0], p.shape[1]), "write_points_csv")
+
+
+def warmup_points(model: "Model", nPts: int, max_iters: int | None = None, tol: float | None = None,
+                  seed: int | None = None, check_every: int | None = None):
+    """Warmup set by flux variability + random-objective LPs on the GPU
+    (achr_warmup_generate). Returns (W: n x nPts, one point per column;
+    fva: n x 2 minimum / maximum per reaction; converged: nPts bools, one per
+    LP; report dict)."""
+    o = AchrWarmupOptions()
+    lib().achr_warmup_options_default(C.byref(o))
+    if max_iters is not None:
+        o.max_iters = max_iters
+    if tol is not None:
+        o.tol = tol
+    if seed is not None:
+        o.seed = seed
+    if check_every is not None:
+        o.check_every = check_every
+    n = model.n
+    W = np.empty((n, nPts), order="F")
+    fva = np.empty((2, n))
+    rep = np.empty(4)
+    conv = np.empty(nPts, dtype=np.int32)
+    check(lib().achr_warmup_generate(model._h, nPts, C.byref(o), W.ctypes.data_as(C.POINTER(C.c_double)), _dp(fva),
+                                     conv.ctypes.data_as(C.POINTER(C.c_int)), _dp(rep)), "achr_warmup_generate")
+    return W, fva.T.copy(), conv.astype(bool), {"iterations": int(rep[0]), "converged": int(rep[1]),
+                                                "ms": float(rep[2]), "scaled_norm": float(rep[3])}
 
 
 def write_points_bin(path: str, points: np.ndarray) -> None:

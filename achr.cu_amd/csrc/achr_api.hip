@@ -292,8 +292,24 @@ int achr_model_read_mps(const char* path, achr_model** out) {
   const int n = static_cast<int>(col_index.size());
   const int m = static_cast<int>(row_index.size());
   if (n == 0) return fail(ACHR_E_FORMAT, "%s: no columns", path);
-  return build_model(n, m, static_cast<int>(vv.size()), rr.data(), cc.data(), vv.data(), lb.data(), ub.data(),
-                     out);
+  rc = build_model(n, m, static_cast<int>(vv.size()), rr.data(), cc.data(), vv.data(), lb.data(), ub.data(), out);
+  if (rc != ACHR_OK) return rc;
+  (*out)->names.resize(n);
+  for (const auto& kv : col_index) (*out)->names[kv.second] = kv.first;
+  return ACHR_OK;
+}
+
+int achr_write_fva_csv(const char* path, const achr_model* M, const double* fva) {
+  if (!path || !M || !fva) return fail(ACHR_E_ARG, "NULL argument");
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(ACHR_E_IO, "cannot open %s for writing", path);
+  std::fprintf(f, "reaction,min,max\n");
+  for (int i = 0; i < M->n; ++i) {
+    const std::string name = i < static_cast<int>(M->names.size()) ? M->names[i] : "?";
+    std::fprintf(f, "%s,%f,%f\n", name.c_str(), fva[i], fva[static_cast<size_t>(M->n) + i]);
+  }
+  if (std::fclose(f) != 0) return fail(ACHR_E_IO, "write error on %s", path);
+  return ACHR_OK;
 }
 
 // -------------------------------------------------------------- null space

@@ -72,6 +72,7 @@ struct achr_model {
   std::vector<int> rowptr, colidx;
   std::vector<double> vals, lb, ub;
   std::vector<double> N;  // n x d column major
+  std::vector<std::string> names;  // reaction (column) names of an MPS model
   int d = -1;             // -1: not computed yet
 };
 

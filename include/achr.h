@@ -329,6 +329,41 @@ int achr_sampler_start_point(const achr_sampler *sampler, double *x0);
  * (ACHRCuda.cu:744-756). */
 int achr_write_points_csv(const char *path, const double *points, int n, int P);
 
+/* ------------------------------------------------------- warmup points */
+
+typedef struct {
+  int max_iters;    /* PDHG iterations at most (default 40,000) */
+  int check_every;  /* iterations between restart / convergence checks (64) */
+  double tol;       /* per LP, on the rescaled problem: |A x| and the duality gap relative to
+                       1 + |objectives| (1e-9) */
+  uint64_t seed;    /* random objectives (1) */
+} achr_warmup_options;
+
+void achr_warmup_options_default(achr_warmup_options *opts);
+
+/* The reference's warmup generator (VFWarmup/createWarmupPtsGLPK.c:67-182,
+ * :283-297) on the device: nPts LPs (nPts even, >= 2n) min / max c^T x
+ * subject to S x = 0, lb <= x <= ub, with c = e_i for the first n pairs (flux
+ * variability) and a random c in [-0.5, 0.5)^n per LP after that, all solved
+ * at once by a batched primal-dual hybrid gradient method (PDLP restarts);
+ * each solution clipped to the bounds and moved toward the points' mean,
+ * x <- 0.33 x + 0.67 mean. warmup: n x nPts column major (each point
+ * contiguous, the sampler's layout). fva (n x 2, may be NULL): the minimum and
+ * maximum of each reaction. converged (nPts ints, may be NULL): 1 for each LP
+ * whose iterate met tol, 0 for one stopped by max_iters (a first-order method:
+ * its value is approximate, its point only approximately on S x = 0).
+ * report (4 doubles, may be NULL): iterations, LPs converged, solve
+ * milliseconds, |A~|_2 of the rescaled matrix. The reference solves one LP at
+ * a time with GLPK's simplex over MPI ranks. */
+int achr_warmup_generate(const achr_model *model, int nPts, const achr_warmup_options *opts, double *warmup,
+                         double *fva, int *converged, double *report);
+
+/* The reference generator's raw FVA file (createWarmupPtsGLPK.c:253-268):
+ * "reaction,min,max" then one "name,%f,%f" line per reaction (names from the
+ * MPS file; "?" for models built from arrays). fva: n x 2 as returned by
+ * achr_warmup_generate. */
+int achr_write_fva_csv(const char *path, const achr_model *model, const double *fva);
+
 /* Lossless counterpart of achr_write_points_csv (whose "%f" keeps six
  * decimals): the 8 bytes "ACHRPT01", int32 n, int32 P, then the n x P
  * doubles, reaction major, little endian. No counterpart in the reference. */

@@ -170,3 +170,19 @@ def test_points_bin_round_trip_is_lossless(tmp_path):
         achr.read_points_bin(tmp_path / "short.bin")
     with pytest.raises(achr.AchrError):
         achr.read_points_bin(tmp_path / "missing.bin")
+
+
+def test_fva_csv_names_and_layout(tmp_path):
+    """The raw FVA file of the warmup generator: header, then one
+    "name,%f,%f" line per reaction with the MPS column names."""
+    model = achr.Model.read_mps(golden("c1.mps"))
+    n = model.n
+    fva = np.stack([-np.arange(n) / 3.0, np.arange(n) * 2.5]).astype(np.float64)  # rows: min, max
+    p = tmp_path / "m_fva_bounds.csv"
+    rc = achr.lib().achr_write_fva_csv(str(p).encode(), model._h, fva.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0
+    lines = p.read_text().splitlines()
+    assert lines[0] == "reaction,min,max" and len(lines) == n + 1
+    names = [l.split(",")[0] for l in lines[1:]]
+    assert len(set(names)) == n and "?" not in names
+    assert lines[1 + 7].split(",")[1:] == ["%f" % (-7 / 3.0), "%f" % 17.5]
