@@ -10,7 +10,10 @@ steps. The default sampler is the learned one: a plain-ACHR training run
 plain FP64 sampler runs the same file for comparison (speedup, moment
 agreement, split-R-hat). "ecoli" is the 72 x 95 fixture model, 500 chains x
 1,000 steps; "batch" is 64 models of 100..13,000 reactions in one job
-(aggregate chain-steps/s).
+(aggregate chain-steps/s); "precision" the projection rounds in FP64, FP32
+and split-BF16x3 on 65,536 chains; "convergence" short runs against a long
+plain run; "warmup" the 21,200 flux-variability LPs of the genome model's
+warmup set (LPs converged per second).
 
 One bench step = one file of the reference CLI (all chains, all steps), with
 inputs already resident in HBM. The timed region is bracketed by device
@@ -400,6 +403,64 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_warmup(args):
+    """Warmup generation for the genome model: the 2n flux-variability LPs of
+    the reference's createWarmupPts (min and max of every reaction subject to
+    S x = 0 and the bounds) solved at once by the batched PDHG solver, with an
+    iteration cap that keeps the run near a minute; value = LPs converged to
+    the solver tolerance per second. CPU baseline: scipy HiGHS (dual simplex,
+    one thread) on a sample of the same LPs."""
+    import achr
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+    m, _ = genome_model()
+    n = m.n
+    model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
+    iters = args.warmup_iters
+    log(f"warmup generation: {2 * n} LPs, at most {iters} iterations")
+    achr.warmup_points(model, 2 * n, max_iters=64)  # untimed: device and code warm
+    t = time.perf_counter()
+    W, fva, conv, rep = achr.warmup_points(model, 2 * n, max_iters=iters)
+    wall = time.perf_counter() - t
+    log(f"warmup: {rep}, wall {wall:.1f}s")
+    S = sp.csr_matrix((m.vals, (m.rows, m.cols)), shape=(m.m, n))
+    # HiGHS on a sample of the same LPs (about a minute each on one core);
+    # its optima also check the converged GPU columns
+    rng = np.random.default_rng(0)
+    sample = rng.choice(2 * n, size=args.cpu_lps, replace=False)
+    cpu, solved, checked, worst = 0.0, 0, 0, 0.0
+    for col in sample:
+        i, sgn = col // 2, (1.0 if col % 2 == 0 else -1.0)
+        c = np.zeros(n)
+        c[i] = sgn
+        log(f"HiGHS: LP {col} ({'min' if sgn > 0 else 'max'} of reaction {i})")
+        t = time.perf_counter()
+        r = linprog(c, A_eq=S, b_eq=np.zeros(m.m), bounds=list(zip(m.lb, m.ub)), method="highs-ds",
+                    options={"time_limit": 170.0})
+        cpu += time.perf_counter() - t
+        log(f"HiGHS: status {r.status} in {time.perf_counter() - t:.1f}s")
+        if r.status != 0:
+            continue
+        solved += 1
+        if conv[col]:
+            checked += 1
+            worst = max(worst, abs(sgn * r.fun - fva[i, col % 2]) / (1 + abs(r.fun)))
+    # an LP that hits the time limit bounds the rate from above
+    cpu_rate = max(solved, 1) / cpu
+    out = {"n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "metric": "flux-variability LPs converged per second (genome model warmup generation)",
+           "value": rep["converged"] / wall, "unit": "LPs/s", "ms_per_step": wall * 1e3,
+           "config": {"workload": f"warmup: {m.m}x{n} model, {2 * n} FVA LPs, batched PDHG, {iters} iterations at most",
+                      "lps": 2 * n, "max_iters": iters},
+           "converged": rep["converged"], "iterations": rep["iterations"], "solve_ms": rep["ms"],
+           "checked_vs_highs": {"lps": checked, "max_rel_err": worst},
+           "cpu_baseline": {"value": cpu_rate, "unit": "LPs/s", "cores": 1, "kind": "port",
+                            "sample": f"scipy HiGHS dual simplex, {solved} of {args.cpu_lps} sampled LPs solved in "
+                                      f"{cpu:.1f}s" + ("" if solved else " (time limit: the value is an upper bound)")}}
+    print(json.dumps(out), flush=True)
+
+
 def run_precision(args):
     """Reduced-precision projection on the genome model with 65,536
     concurrent chains: the projection rounds of every 10th step in FP64 (the
@@ -580,7 +641,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed files (bench steps)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed files (short: 20 chain steps)")
-    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision", "convergence"], default="genome")
+    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision", "convergence", "warmup"],
+                    default="genome")
+    ap.add_argument("--warmup-iters", type=int, default=8000, help="warmup workload: PDHG iteration cap")
+    ap.add_argument("--cpu-lps", type=int, default=1, help="warmup workload: LPs HiGHS times (about 1 min each)")
     ap.add_argument("--ref-steps", type=int, default=20000, help="convergence workload: steps of the reference run")
     ap.add_argument("--models", type=int, default=64, help="batch workload: number of models")
     ap.add_argument("--probe-steps", type=int, default=20, help="batch workload: autotuner probe length")
@@ -621,6 +685,9 @@ def main():
         return
     if args.workload == "convergence":
         run_convergence(args)
+        return
+    if args.workload == "warmup":
+        run_warmup(args)
         return
     if args.workload == "batch":
         if args.cpu_chains == 32 and args.cpu_steps == 300:
