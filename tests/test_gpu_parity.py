@@ -172,9 +172,10 @@ def test_cli_drop_in(tmp_path, c1_setup):
 
 @pytest.mark.parametrize("mets,rxns,lanes", [(4900, 9000, 1024), (3800, 7000, 512)])
 def test_genome_scale_schedule_bitwise(mets, rxns, lanes):
-    """Genome-scale sizes: 9 (padded to 12) or 14 (padded to 16) slots per
-    thread with the centre in LDS (96 KiB / 64.5 KiB of dynamic LDS), batched
-    FP64 projection on a null space from the GPU solver."""
+    """Genome-scale sizes: 9 slots per thread at 1,024 lanes (every slot
+    count has a kernel there) or 14 padded to 16 at 512 lanes, with the
+    centre in LDS (72 KiB / 64 KiB of dynamic LDS), batched FP64 projection
+    on a null space from the GPU solver."""
     from achr import synth
 
     def ns(rows, cols, vals, m, n):
