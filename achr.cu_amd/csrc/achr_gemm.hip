@@ -111,12 +111,22 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
   const TA* A = static_cast<const TA*>(g.A);
   const bool a_kc = g.sa_k == 1;
   const bool b_kc = g.sb_k == 1;
+  // BF16 paths load the next K-slice into registers while the matrix cores
+  // work on the current one (issued right after the LDS stores: -4% per
+  // residual round); the FP32 path loads each slice at the top of its
+  // iteration (the early load made it 4% slower).
+  float va[16], vb[16];
+  int ra, ka, rb, kb;
+  bool ala, alb;
+  load_tile<TA>(A, g.sa_m, g.sa_k, a_kc, g.idxA, i0, M, 0, g.K, va, ra, ka, ala);
+  load_tile<float>(g.B, g.sb_n, g.sb_k, b_kc, nullptr, j0, g.N, 0, g.K, vb, rb, kb, alb);
   for (int k0 = 0; k0 < g.K; k0 += BK) {
-    float va[16], vb[16];
-    int ra, ka, rb, kb;
-    bool ala, alb;
-    load_tile<TA>(A, g.sa_m, g.sa_k, a_kc, g.idxA, i0, M, k0, g.K, va, ra, ka, ala);
-    load_tile<float>(g.B, g.sb_n, g.sb_k, b_kc, nullptr, j0, g.N, k0, g.K, vb, rb, kb, alb);
+    if constexpr (!kBF) {
+      if (k0 > 0) {
+        load_tile<TA>(A, g.sa_m, g.sa_k, a_kc, g.idxA, i0, M, k0, g.K, va, ra, ka, ala);
+        load_tile<float>(g.B, g.sb_n, g.sb_k, b_kc, nullptr, j0, g.N, k0, g.K, vb, rb, kb, alb);
+      }
+    }
     __syncthreads();  // previous tile fully consumed
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
@@ -137,6 +147,12 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
       }
     }
     __syncthreads();
+    if constexpr (kBF) {
+      if (k0 + BK < g.K) {
+        load_tile<TA>(A, g.sa_m, g.sa_k, a_kc, g.idxA, i0, M, k0 + BK, g.K, va, ra, ka, ala);
+        load_tile<float>(g.B, g.sb_n, g.sb_k, b_kc, nullptr, j0, g.N, k0 + BK, g.K, vb, rb, kb, alb);
+      }
+    }
     if constexpr (kBF) {
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 16) {
