@@ -1032,7 +1032,20 @@ struct StateHeader {
 constexpr char kStateMagic[8] = {'A', 'C', 'H', 'R', 'S', 'T', '0', '1'};
 }  // namespace
 
+static int save_state(const achr_sampler* S, const char* path);
+static int load_state(achr_sampler* S, const char* path);
+
 int achr_sampler_save_state(const achr_sampler* S, const char* path) {
+  return achr_host::no_throw([&] { return save_state(S, path); });
+}
+
+int achr_sampler_load_state(achr_sampler* S, const char* path) {
+  return achr_host::no_throw([&] { return load_state(S, path); });
+}
+
+}  // extern "C"
+
+static int save_state(const achr_sampler* S, const char* path) {
   if (!S || !path) return fail(ACHR_E_ARG, "NULL argument");
   if (!S->have_run) return fail(ACHR_E_STATE, "no run");
   HIP_TRY(hipEventSynchronize(S->ev_end));
@@ -1068,7 +1081,7 @@ int achr_sampler_save_state(const achr_sampler* S, const char* path) {
   return ACHR_OK;
 }
 
-int achr_sampler_load_state(achr_sampler* S, const char* path) {
+static int load_state(achr_sampler* S, const char* path) {
   if (!S || !path) return fail(ACHR_E_ARG, "NULL argument");
   FILE* f = std::fopen(path, "rb");
   if (!f) return fail(ACHR_E_IO, "cannot read %s: %s", path, std::strerror(errno));
@@ -1080,7 +1093,7 @@ int achr_sampler_load_state(achr_sampler* S, const char* path) {
   const int warm_count = S->use_dirs ? S->K : S->W;
   if (h.n != S->n || h.lanes != S->sched.tpc || h.use_dirs != S->use_dirs || h.warm_count != warm_count ||
       h.check_every != S->opt.check_every || h.mode != S->mode || h.model_id != S->opt.model_id || h.P <= 0 ||
-      h.S < 0 || h.chain0 < 0) {
+      h.P > (1 << 28) || h.S < 0 || h.chain0 < 0) {
     std::fclose(f);
     return fail(ACHR_E_STATE, "%s was saved by a sampler with other settings (n, lanes, directions, warmup or bank "
                 "size, check period, reprojection mode or model_id)", path);
@@ -1120,6 +1133,8 @@ int achr_sampler_load_state(achr_sampler* S, const char* path) {
   S->have_run = true;
   return ACHR_OK;
 }
+
+extern "C" {
 
 int achr_sample_points(achr_sampler* S, double* out) {
   if (!S || !out) return fail(ACHR_E_ARG, "NULL argument");
@@ -1270,6 +1285,10 @@ int achr_read_points_bin(const char* path, double** points, int* n, int* P) {
     return fail(ACHR_E_FORMAT, "%s is not a point file", path);
   }
   const size_t cnt = static_cast<size_t>(dims[0]) * dims[1];
+  if (cnt > (static_cast<size_t>(1) << 40)) {
+    std::fclose(f);
+    return fail(ACHR_E_FORMAT, "%s: %d x %d points is not a plausible size", path, dims[0], dims[1]);
+  }
   double* buf = static_cast<double*>(std::malloc(sizeof(double) * cnt));
   if (!buf) {
     std::fclose(f);

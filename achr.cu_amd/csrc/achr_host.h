@@ -5,6 +5,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <new>
+#include <stdexcept>
 #include <utility>
 #include <vector>
 
@@ -116,6 +118,19 @@ struct achr_sampler {
 };
 
 namespace achr_host {
+
+// Runs f() and turns a host allocation failure into ACHR_E_NOMEM, for entry
+// points whose host buffers are sized by their inputs (files, models).
+template <class F>
+int no_throw(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(ACHR_E_NOMEM, "out of host memory");
+  } catch (const std::length_error&) {
+    return fail(ACHR_E_NOMEM, "host buffer too large");
+  }
+}
 
 // Run set-up and projection rounds shared by single-model runs and batches.
 enum { kProjNone = 0, kProjFP64 = 1, kProjLow = 2 };

@@ -294,7 +294,7 @@ int achr_sampler_use_directions(achr_sampler* S, int on) {
   if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
   if (on && S->K == 0) return fail(ACHR_E_STATE, "no direction bank");
   if (on && S->s0.empty()) {
-    const int rc = compute_bank_start(S);
+    const int rc = achr_host::no_throw([&] { return compute_bank_start(S); });
     if (rc != ACHR_OK) return rc;
   }
   S->use_dirs = on ? 1 : 0;

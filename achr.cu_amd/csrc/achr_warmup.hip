@@ -262,6 +262,9 @@ __global__ __launch_bounds__(256) void count_done_kernel(int B, const int* __res
 
 unsigned blocks_for(size_t count) { return static_cast<unsigned>((count + 255) / 256); }
 
+int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* opts, double* warmup, double* fva,
+                    int* converged, double* report);
+
 }  // namespace
 
 extern "C" {
@@ -276,6 +279,15 @@ void achr_warmup_options_default(achr_warmup_options* o) {
 
 int achr_warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* opts, double* warmup, double* fva,
                          int* converged, double* report) {
+  return achr_host::no_throw([&] { return warmup_generate(M, nPts, opts, warmup, fva, converged, report); });
+}
+
+}  // extern "C"
+
+namespace {
+
+int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* opts, double* warmup, double* fva,
+                    int* converged, double* report) {
   if (!M || !warmup) return fail(ACHR_E_ARG, "NULL argument");
   achr_warmup_options o;
   achr_warmup_options_default(&o);
@@ -496,4 +508,4 @@ int achr_warmup_generate(const achr_model* M, int nPts, const achr_warmup_option
   return ACHR_OK;
 }
 
-}  // extern "C"
+}  // namespace

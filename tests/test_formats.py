@@ -170,6 +170,9 @@ def test_points_bin_round_trip_is_lossless(tmp_path):
         achr.read_points_bin(tmp_path / "short.bin")
     with pytest.raises(achr.AchrError):
         achr.read_points_bin(tmp_path / "missing.bin")
+    (tmp_path / "huge.bin").write_bytes(b"ACHRPT01" + np.array([2**31 - 1, 2**31 - 1], "<i4").tobytes())
+    with pytest.raises(achr.AchrError):
+        achr.read_points_bin(tmp_path / "huge.bin")  # refused before allocating
 
 
 def test_fva_csv_names_and_layout(tmp_path):
