@@ -38,6 +38,9 @@ SIGNATURES = {
     "achr_version": (C.c_char_p, []),
     "achr_model_create": (_I, [_I, _I, _I, _PI, _PI, _PD, _PD, _PD, C.POINTER(_P)]),
     "achr_model_read_mps": (_I, [C.c_char_p, C.POINTER(_P)]),
+    "achr_model_read_mps_rows": (_I, [C.c_char_p, C.POINTER(_P)]),
+    "achr_model_file_columns": (_I, [_P, _PI]),
+    "achr_model_complete_points": (_I, [_P, _PD, _I, _PD]),
     "achr_model_destroy": (None, [_P]),
     "achr_model_dims": (_I, [_P, _PI, _PI, _PI, _PI]),
     "achr_model_bounds": (_I, [_P, _PD, _PD]),

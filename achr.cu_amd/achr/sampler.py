@@ -52,10 +52,35 @@ class Model:
         return cls.from_coo(S.shape[1], S.shape[0], r, c, S[r, c], lb, ub)
 
     @classmethod
-    def read_mps(cls, path: str) -> "Model":
+    def read_mps(cls, path: str, rows: bool = False) -> "Model":
+        """rows=False: the reference sampler's view (every row an equality with
+        right-hand side 0). rows=True: row types, right-hand sides and ranges
+        as written, carried by auxiliary columns after the file's own."""
         h = C.c_void_p()
-        check(lib().achr_model_read_mps(str(path).encode(), C.byref(h)), "achr_model_read_mps")
+        fn = "achr_model_read_mps_rows" if rows else "achr_model_read_mps"
+        check(getattr(lib(), fn)(str(path).encode(), C.byref(h)), fn)
         return cls(h)
+
+    @property
+    def file_columns(self) -> int:
+        """Leading columns that come from the file (n unless read with rows=True)."""
+        nf = C.c_int()
+        check(lib().achr_model_file_columns(self._h, C.byref(nf)), "achr_model_file_columns")
+        return nf.value
+
+    def complete_points(self, X_file: np.ndarray) -> np.ndarray:
+        """(n_file x P) points of the file's columns -> (n x P) with the
+        auxiliary columns filled: slacks take their row's activity, the
+        right-hand-side column 1."""
+        X_file = np.asarray(X_file, dtype=np.float64)
+        nf, n = self.file_columns, self.n
+        if X_file.ndim != 2 or X_file.shape[0] != nf:
+            raise ValueError(f"expected {nf} rows")
+        P = X_file.shape[1]
+        xi = np.ascontiguousarray(X_file.T)
+        xo = np.empty((P, n))
+        check(lib().achr_model_complete_points(self._h, _dp(xi), P, _dp(xo)), "achr_model_complete_points")
+        return np.ascontiguousarray(xo.T)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -105,6 +105,7 @@ static int build_model(int n, int m, int nnz, const int* rows, const int* cols, 
   }
   M->lb.assign(lb, lb + n);
   M->ub.assign(ub, ub + n);
+  M->n_file = n;
   *out = M;
   return ACHR_OK;
 }
@@ -163,18 +164,23 @@ static bool parse_num(const std::string& s, double* v) {
   return end && *end == '\0' && errno != ERANGE;
 }
 
-int achr_model_read_mps(const char* path, achr_model** out) {
-  if (!path || !out) return fail(ACHR_E_ARG, "NULL argument");
-  *out = nullptr;
+// What an MPS file holds, before either view of its rows is taken.
+struct MpsFile {
+  std::vector<std::string> row_names, col_names;
+  std::vector<char> row_type;  // 'E', 'L', 'G', or 'N' for a free row after the objective
+  std::vector<double> rhs, range;  // range: NaN where the RANGES section gives none
+  std::vector<int> rr, cc;
+  std::vector<double> vv, lb, ub;
+};
+
+static int parse_mps(const char* path, MpsFile& F) {
+  if (!path) return fail(ACHR_E_ARG, "NULL argument");
   FILE* f = std::fopen(path, "r");
   if (!f) return fail(ACHR_E_IO, "cannot open MPS file %s", path);
   enum Sec { NONE, ROWS, COLUMNS, RHS, RANGES, BOUNDS, DONE } sec = NONE;
   std::map<std::string, int> row_index;  // constraint rows only
   std::string obj_row;
   std::map<std::string, int> col_index;
-  std::vector<int> rr, cc;
-  std::vector<double> vv;
-  std::vector<double> lb, ub;
   std::vector<std::string> tok;
   std::vector<char> buf(1 << 16);
   int lineno = 0;
@@ -183,8 +189,9 @@ int achr_model_read_mps(const char* path, achr_model** out) {
     if (it != col_index.end()) return it->second;
     const int j = static_cast<int>(col_index.size());
     col_index.emplace(name, j);
-    lb.push_back(0.0);
-    ub.push_back(DBL_MAX);
+    F.col_names.push_back(name);
+    F.lb.push_back(0.0);
+    F.ub.push_back(DBL_MAX);
     return j;
   };
   int rc = ACHR_OK;
@@ -216,6 +223,10 @@ int achr_model_read_mps(const char* path, achr_model** out) {
       }
       const int i = static_cast<int>(row_index.size());
       row_index.emplace(tok[1], i);
+      F.row_names.push_back(tok[1]);
+      F.row_type.push_back(type[0]);
+      F.rhs.push_back(0.0);
+      F.range.push_back(NAN);
     } else if (sec == COLUMNS) {
       if (tok.size() >= 3 && (tok[1] == "'MARKER'" || tok[2] == "'MARKER'")) continue;
       if (tok.size() != 3 && tok.size() != 5) {
@@ -232,9 +243,29 @@ int achr_model_read_mps(const char* path, achr_model** out) {
           rc = fail(ACHR_E_FORMAT, "%s:%d: unknown row %s", path, lineno, tok[k].c_str());
           break;
         }
-        rr.push_back(it->second);
-        cc.push_back(j);
-        vv.push_back(v);
+        F.rr.push_back(it->second);
+        F.cc.push_back(j);
+        F.vv.push_back(v);
+      }
+      if (rc != ACHR_OK) break;
+    } else if (sec == RHS || sec == RANGES) {
+      // [set name] row value [row value]: the set name is absent when the
+      // line has an even number of fields.
+      const size_t first = tok.size() % 2 == 0 ? 0 : 1;
+      if (tok.size() < 2 || tok.size() > 5) {
+        rc = fail(ACHR_E_FORMAT, "%s:%d: bad %s line", path, lineno, sec == RHS ? "RHS" : "RANGES");
+        break;
+      }
+      for (size_t k = first; k + 1 < tok.size(); k += 2) {
+        double v;
+        if (!parse_num(tok[k + 1], &v)) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad number", path, lineno); break; }
+        if (tok[k] == obj_row) continue;  // minus the objective's constant term
+        auto it = row_index.find(tok[k]);
+        if (it == row_index.end()) {
+          rc = fail(ACHR_E_FORMAT, "%s:%d: unknown row %s", path, lineno, tok[k].c_str());
+          break;
+        }
+        (sec == RHS ? F.rhs : F.range)[it->second] = v;
       }
       if (rc != ACHR_OK) break;
     } else if (sec == BOUNDS) {
@@ -257,45 +288,201 @@ int achr_model_read_mps(const char* path, achr_model** out) {
         break;
       }
       const int j = it->second;
+      double& lo = F.lb[j];
+      double& hi = F.ub[j];
       if (type == "UP") {
-        ub[j] = v;
-        if (v < 0.0 && lb[j] == 0.0) lb[j] = -DBL_MAX;  // MPS convention for a negative UP bound
+        hi = v;
+        if (v < 0.0 && lo == 0.0) lo = -DBL_MAX;  // MPS convention for a negative UP bound
       } else if (type == "LO") {
-        lb[j] = v;
+        lo = v;
       } else if (type == "FX") {
-        lb[j] = v;
-        ub[j] = v;
+        lo = v;
+        hi = v;
       } else if (type == "FR") {
-        lb[j] = -DBL_MAX;
-        ub[j] = DBL_MAX;
+        lo = -DBL_MAX;
+        hi = DBL_MAX;
       } else if (type == "MI") {
-        lb[j] = -DBL_MAX;
+        lo = -DBL_MAX;
       } else if (type == "PL") {
-        ub[j] = DBL_MAX;
+        hi = DBL_MAX;
       } else if (type == "BV") {
-        lb[j] = 0.0;
-        ub[j] = 1.0;
+        lo = 0.0;
+        hi = 1.0;
       } else if (type == "LI") {
-        lb[j] = v;
+        lo = v;
       } else if (type == "UI") {
-        ub[j] = v;
+        hi = v;
       } else {
         rc = fail(ACHR_E_FORMAT, "%s:%d: bad bound type %s", path, lineno, type.c_str());
         break;
       }
     }
-    // RHS and RANGES are ignored, as the reference ignores them (it samples S x = 0).
   }
   std::fclose(f);
   if (rc != ACHR_OK) return rc;
   if (sec != DONE) return fail(ACHR_E_FORMAT, "%s: missing ENDATA", path);
-  const int n = static_cast<int>(col_index.size());
-  const int m = static_cast<int>(row_index.size());
-  if (n == 0) return fail(ACHR_E_FORMAT, "%s: no columns", path);
+  if (F.col_names.empty()) return fail(ACHR_E_FORMAT, "%s: no columns", path);
+  return ACHR_OK;
+}
+
+// The reference's view (ACHRCuda.cu:563-636): every row after the objective,
+// whatever its type, is a row of S, and the sampled set is S x = 0; RHS and
+// RANGES are read and ignored.
+int achr_model_read_mps(const char* path, achr_model** out) {
+  if (!path || !out) return fail(ACHR_E_ARG, "NULL argument");
+  *out = nullptr;
+  MpsFile F;
+  int rc = parse_mps(path, F);
+  if (rc != ACHR_OK) return rc;
+  const int n = static_cast<int>(F.col_names.size());
+  const int m = static_cast<int>(F.row_names.size());
+  rc = build_model(n, m, static_cast<int>(F.vv.size()), F.rr.data(), F.cc.data(), F.vv.data(), F.lb.data(),
+                   F.ub.data(), out);
+  if (rc != ACHR_OK) return rc;
+  (*out)->names = std::move(F.col_names);
+  return ACHR_OK;
+}
+
+// The rows as written, the view GLPK gives the reference's warmup generator
+// (createWarmupPtsGLPK.c:219): an E row is S_i x = b_i, an L row S_i x <= b_i,
+// a G row S_i x >= b_i, RANGES give the other side (MPS convention: E with
+// R > 0 is [b, b + R], with R < 0 [b + R, b]; L is [b - |R|, b]; G is
+// [b, b + |R|]), free rows after the objective constrain nothing. The result
+// is again a set {x' : S' x' = 0, lb' <= x' <= ub'} whose first n_file
+// coordinates range over exactly the polytope of the file:
+//   - a row with lo < hi gets a slack column s_i with S_i x - s_i = 0 and
+//     bounds [lo, hi], an infinite side replaced by the row's activity range
+//     over the column bounds (which the row can never leave);
+//   - an equality row with b_i != 0 gets the coefficient -b_i on one column
+//     fixed at 1, appended last.
+int achr_model_read_mps_rows(const char* path, achr_model** out) {
+  if (!path || !out) return fail(ACHR_E_ARG, "NULL argument");
+  *out = nullptr;
+  MpsFile F;
+  int rc = parse_mps(path, F);
+  if (rc != ACHR_OK) return rc;
+  const int n_file = static_cast<int>(F.col_names.size());
+  const int m_file = static_cast<int>(F.row_names.size());
+  constexpr double kInf = 1e30;  // bounds at or beyond this are infinite
+  std::vector<double> act_lo(m_file, 0.0), act_hi(m_file, 0.0);
+  for (size_t k = 0; k < F.vv.size(); ++k) {
+    const double a = F.vv[k], l = F.lb[F.cc[k]], u = F.ub[F.cc[k]];
+    if (a == 0.0) continue;
+    const double lo_term = a > 0 ? (l <= -kInf ? -INFINITY : a * l) : (u >= kInf ? -INFINITY : a * u);
+    const double hi_term = a > 0 ? (u >= kInf ? INFINITY : a * u) : (l <= -kInf ? INFINITY : a * l);
+    act_lo[F.rr[k]] += lo_term;
+    act_hi[F.rr[k]] += hi_term;
+  }
+  std::vector<int> row_new(m_file, -1), aux_row;
+  std::vector<double> aux_lb, aux_ub, rhs_coef(m_file, 0.0);
+  std::vector<std::string> aux_name;
+  int m = 0;
+  bool need_one = false;
+  for (int i = 0; i < m_file; ++i) {
+    const char t = F.row_type[i];
+    if (t == 'N') continue;  // a free row
+    const double b = F.rhs[i], R = F.range[i];
+    const bool ranged = !std::isnan(R);
+    double lo = -INFINITY, hi = INFINITY;
+    if (t == 'E') {
+      lo = hi = b;
+      if (ranged && R > 0) hi = b + R;
+      if (ranged && R < 0) lo = b + R;
+    } else if (t == 'L') {
+      hi = b;
+      if (ranged) lo = b - std::fabs(R);
+    } else {
+      lo = b;
+      if (ranged) hi = b + std::fabs(R);
+    }
+    row_new[i] = m++;
+    if (lo == hi) {
+      if (b != 0.0) {
+        rhs_coef[i] = -b;
+        need_one = true;
+      }
+      continue;
+    }
+    lo = std::max(lo, act_lo[i]);
+    hi = std::min(hi, act_hi[i]);
+    if (!std::isfinite(lo) || !std::isfinite(hi))
+      return fail(ACHR_E_FORMAT, "%s: row %s is unbounded %s within the column bounds; ACHR needs a bounded set",
+                  path, F.row_names[i].c_str(), std::isfinite(lo) ? "above" : "below");
+    if (lo > hi)
+      return fail(ACHR_E_FORMAT, "%s: row %s cannot be met within the column bounds ([%g, %g])", path,
+                  F.row_names[i].c_str(), lo, hi);
+    aux_row.push_back(row_new[i]);
+    aux_lb.push_back(lo);
+    aux_ub.push_back(hi);
+    aux_name.push_back("slack:" + F.row_names[i]);
+  }
+  const int n_slack = static_cast<int>(aux_row.size());
+  const int n = n_file + n_slack + (need_one ? 1 : 0);
+  std::vector<int> rr, cc;
+  std::vector<double> vv;
+  for (size_t k = 0; k < F.vv.size(); ++k) {
+    if (row_new[F.rr[k]] < 0) continue;
+    rr.push_back(row_new[F.rr[k]]);
+    cc.push_back(F.cc[k]);
+    vv.push_back(F.vv[k]);
+  }
+  for (int k = 0; k < n_slack; ++k) {
+    rr.push_back(aux_row[k]);
+    cc.push_back(n_file + k);
+    vv.push_back(-1.0);
+  }
+  if (need_one) {
+    for (int i = 0; i < m_file; ++i)
+      if (rhs_coef[i] != 0.0) {
+        rr.push_back(row_new[i]);
+        cc.push_back(n - 1);
+        vv.push_back(rhs_coef[i]);
+      }
+    aux_row.push_back(-1);
+    aux_lb.push_back(1.0);
+    aux_ub.push_back(1.0);
+    aux_name.push_back("rhs:1");
+  }
+  std::vector<double> lb(F.lb), ub(F.ub);
+  lb.insert(lb.end(), aux_lb.begin(), aux_lb.end());
+  ub.insert(ub.end(), aux_ub.begin(), aux_ub.end());
   rc = build_model(n, m, static_cast<int>(vv.size()), rr.data(), cc.data(), vv.data(), lb.data(), ub.data(), out);
   if (rc != ACHR_OK) return rc;
-  (*out)->names.resize(n);
-  for (const auto& kv : col_index) (*out)->names[kv.second] = kv.first;
+  achr_model* M = *out;
+  M->names = std::move(F.col_names);
+  M->names.insert(M->names.end(), aux_name.begin(), aux_name.end());
+  M->n_file = n_file;
+  M->aux_row = std::move(aux_row);
+  return ACHR_OK;
+}
+
+int achr_model_file_columns(const achr_model* M, int* n_file) {
+  if (!M || !n_file) return fail(ACHR_E_ARG, "NULL argument");
+  *n_file = M->n_file;
+  return ACHR_OK;
+}
+
+// Slack k of row r takes the row's activity over the file's columns (CSR
+// order, the slack's own entry skipped); the right-hand-side column is 1.
+int achr_model_complete_points(const achr_model* M, const double* x_file, int P, double* x) {
+  if (!M || (P > 0 && (!x_file || !x))) return fail(ACHR_E_ARG, "NULL argument");
+  if (P < 0) return fail(ACHR_E_ARG, "P=%d", P);
+  const int nf = M->n_file, n = M->n;
+  for (int p = 0; p < P; ++p) {
+    const double* xi = x_file + static_cast<size_t>(p) * nf;
+    double* xo = x + static_cast<size_t>(p) * n;
+    std::copy(xi, xi + nf, xo);
+    for (int k = 0; k < n - nf; ++k) {
+      const int r = M->aux_row[k];
+      double s = 1.0;
+      if (r >= 0) {
+        s = 0.0;
+        for (int q = M->rowptr[r]; q < M->rowptr[r + 1]; ++q)
+          if (M->colidx[q] < nf) s += M->vals[q] * xi[M->colidx[q]];
+      }
+      xo[nf + k] = s;
+    }
+  }
   return ACHR_OK;
 }
 
@@ -304,7 +491,7 @@ int achr_write_fva_csv(const char* path, const achr_model* M, const double* fva)
   FILE* f = std::fopen(path, "w");
   if (!f) return fail(ACHR_E_IO, "cannot open %s for writing", path);
   std::fprintf(f, "reaction,min,max\n");
-  for (int i = 0; i < M->n; ++i) {
+  for (int i = 0; i < M->n_file; ++i) {
     const std::string name = i < static_cast<int>(M->names.size()) ? M->names[i] : "?";
     std::fprintf(f, "%s,%f,%f\n", name.c_str(), fva[i], fva[static_cast<size_t>(M->n) + i]);
   }
@@ -623,7 +810,7 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
   return ACHR_OK;
 }
 
-int achr_read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_out) {
+static int read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_out, int* n_out) {
   if (!path || !warmup || !W_out) return fail(ACHR_E_ARG, "NULL argument");
   *warmup = nullptr;
   FILE* f = std::fopen(path, "r");
@@ -677,17 +864,40 @@ int achr_read_warmup_csv(const char* path, int n_expected, double** warmup, int*
     for (int k = 0; k < W; ++k) w[static_cast<size_t>(k) * n + i] = rows[i][k];
   *warmup = w;
   *W_out = W;
+  if (n_out) *n_out = n;
   return ACHR_OK;
+}
+
+int achr_read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_out) {
+  return read_warmup_csv(path, n_expected, warmup, W_out, nullptr);
 }
 
 int achr_sampler_create_csv(const achr_model* M, const char* path, const achr_options* opts,
                             achr_sampler** out) {
   if (!M) return fail(ACHR_E_ARG, "model is NULL");
   double* w = nullptr;
-  int W = 0;
-  int rc = achr_read_warmup_csv(path, M->n, &w, &W);
+  int W = 0, n = 0;
+  // A model with auxiliary columns also takes a file of its file columns
+  // only (what the reference's generator writes); the rest are completed.
+  int rc = read_warmup_csv(path, M->n_file < M->n ? 0 : M->n, &w, &W, &n);
   if (rc != ACHR_OK) return rc;
-  rc = achr_sampler_create(M, w, W, opts, out);
+  if (n != M->n && n != M->n_file) {
+    std::free(w);
+    return fail(ACHR_E_FORMAT, "%s: %d lines but the model has %d columns (%d in the file)", path, n, M->n, M->n_file);
+  }
+  if (n != M->n) {
+    std::vector<double> full;
+    try {
+      full.resize(static_cast<size_t>(M->n) * W);
+    } catch (const std::bad_alloc&) {
+      std::free(w);
+      return fail(ACHR_E_NOMEM, "warmup allocation");
+    }
+    achr_model_complete_points(M, w, W, full.data());
+    rc = achr_sampler_create(M, full.data(), W, opts, out);
+  } else {
+    rc = achr_sampler_create(M, w, W, opts, out);
+  }
   std::free(w);
   return rc;
 }

@@ -76,6 +76,11 @@ struct achr_model {
   std::vector<double> N;  // n x d column major
   std::vector<std::string> names;  // reaction (column) names of an MPS model
   int d = -1;             // -1: not computed yet
+  // Columns n_file..n-1 of a model read with its rows honoured are auxiliary:
+  // aux_row[k] is the row whose activity column n_file + k carries (its slack),
+  // or -1 for the column fixed at 1 that carries the nonzero right-hand sides.
+  int n_file = 0;
+  std::vector<int> aux_row;
 };
 
 struct achr_sampler {

@@ -292,8 +292,11 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
   achr_warmup_options o;
   achr_warmup_options_default(&o);
   if (opts) o = *opts;
-  const int n = M->n, m = M->m;
-  if (nPts < 2 * n) return fail(ACHR_E_ARG, "nPts=%d is below 2 n = %d (one min and one max per reaction)", nPts, 2 * n);
+  // FVA and random objectives range over the file's columns; the auxiliary
+  // columns of a model read with its rows honoured only carry its rows.
+  const int n = M->n, m = M->m, nf = M->n_file;
+  if (nPts < 2 * nf)
+    return fail(ACHR_E_ARG, "nPts=%d is below 2 n = %d (one min and one max per reaction)", nPts, 2 * nf);
   if (nPts % 2) return fail(ACHR_E_ARG, "nPts=%d must be even (a min and a max per objective)", nPts);
   if (o.max_iters <= 0 || o.check_every <= 0 || !(o.tol > 0.0)) return fail(ACHR_E_ARG, "bad warmup options");
   const int B = nPts;
@@ -370,10 +373,10 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
     us[i] = M->ub[i] / dc[i];
   }
   // random objectives (one per column beyond the FVA pairs), Philox U(-0.5, 0.5)
-  const int R = B - 2 * n;
+  const int R = B - 2 * nf;
   std::vector<double> Cr(static_cast<size_t>(n) * std::max(R, 1), 0.0);
   for (int r = 0; r < R; ++r)
-    for (int i = 0; i < n; ++i)
+    for (int i = 0; i < nf; ++i)
       Cr[static_cast<size_t>(i) * R + r] =
           achr::stream_u01(o.seed, static_cast<uint32_t>(i), static_cast<uint32_t>(r), 0, 0, kTagObjective) - 0.5;
 
@@ -429,7 +432,7 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
   HIP_TRY(hipMemset(d_since.p, 0, sizeof(int) * B));
   HIP_TRY(hipMemset(d_action.p, 0, sizeof(int) * B));
 
-  Lp p{n, m, B, n, d_rp.p, d_ci.p, d_va.p, d_cp.p, d_ri.p, d_vc.p, d_l.p, d_u.p, d_dc.p, d_Cr.p};
+  Lp p{n, m, B, nf, d_rp.p, d_ci.p, d_va.p, d_cp.p, d_ri.p, d_vc.p, d_l.p, d_u.p, d_dc.p, d_Cr.p};
   ColState cs{w.p, tau.p, sig.p, kr.p, kprev.p, den.p, d_since.p, d_done.p, d_action.p};
   const dim3 kb(256), rg(blocks_for(B), rchunks), cg(blocks_for(B), cchunks);
   auto kkt = [&]() {
@@ -488,8 +491,8 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
       row[b] = v;
     }
     if (fva) {
-      fva[i] = row[2 * i];
-      fva[static_cast<size_t>(n) + i] = row[2 * i + 1];
+      fva[i] = i < nf ? row[2 * i] : NAN;
+      fva[static_cast<size_t>(n) + i] = i < nf ? row[2 * i + 1] : NAN;
     }
     double sum = 0.0;
     for (int b = 0; b < B; ++b) sum += row[b];

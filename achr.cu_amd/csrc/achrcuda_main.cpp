@@ -17,6 +17,10 @@
 //   ACHR_REPROJECT=1|2     1 = per chain in the step kernel, 2 = batched GEMM
 //   ACHR_NULLSPACE=<file>  read N (n x d, column major, raw little-endian
 //                          doubles) instead of computing it
+//   ACHR_MPS_ROWS=1        honour the file's row types, right-hand sides and
+//                          ranges (the set createWarmupPts solves over) instead
+//                          of sampling S x = 0 over every row as the reference
+//                          does; File<k> still holds the file's columns only
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -52,9 +56,14 @@ int main(int argc, char** argv) {
   std::printf("\nThe model supplied is %s\n", argv[1]);
 
   achr_model* model = nullptr;
-  if (achr_model_read_mps(argv[1], &model) != ACHR_OK) return die("reading the MPS model");
-  int n = 0, m = 0, nnz = 0, d = 0;
+  const char* rows_env = std::getenv("ACHR_MPS_ROWS");
+  const bool rows = rows_env && std::strcmp(rows_env, "1") == 0;
+  if ((rows ? achr_model_read_mps_rows(argv[1], &model) : achr_model_read_mps(argv[1], &model)) != ACHR_OK)
+    return die("reading the MPS model");
+  int n = 0, m = 0, nnz = 0, d = 0, nf = 0;
   achr_model_dims(model, &n, &m, &nnz, nullptr);
+  achr_model_file_columns(model, &nf);
+  if (nf < n) std::printf("%d slack / right-hand-side columns carry the rows' types and right-hand sides\n", n - nf);
   std::printf("nRxns %d\nnMets %d\nnnz is %d \n", n, m, nnz);
 
   if (const char* npath = std::getenv("ACHR_NULLSPACE")) {
@@ -97,9 +106,10 @@ int main(int argc, char** argv) {
       return die("sampling");
     if (achr_sample_points(sampler, pts.data()) != ACHR_OK) return die("fetching points");
     const std::string name = "File" + std::to_string(k);
-    if (achr_write_points_csv(name.c_str(), pts.data(), n, static_cast<int>(P)) != ACHR_OK)
+    // reaction major: the file's columns are the first nf rows
+    if (achr_write_points_csv(name.c_str(), pts.data(), nf, static_cast<int>(P)) != ACHR_OK)
       return die("writing points");
-    if (lossless && achr_write_points_bin((name + ".bin").c_str(), pts.data(), n, static_cast<int>(P)) != ACHR_OK)
+    if (lossless && achr_write_points_bin((name + ".bin").c_str(), pts.data(), nf, static_cast<int>(P)) != ACHR_OK)
       return die("writing lossless points");
   }
   const double t_all = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

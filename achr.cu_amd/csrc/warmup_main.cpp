@@ -9,8 +9,11 @@
 // its ".mps"). nPts below 2 n is raised to 2 n and an odd nPts to the next
 // even number (a minimum and a maximum per objective). The third argument -1
 // (the reference's GLPK scaling switch) is accepted; the GPU solver always
-// rescales. Environment: ACHR_SEED (random objectives), ACHR_WARMUP_ITERS
-// (iteration cap, default 40,000), ACHR_WARMUP_TOL (default 1e-9).
+// rescales. The LPs honour the file's row types, right-hand sides and ranges
+// as GLPK does for the reference (an E / L / G row is S_i x = / <= / >= b_i);
+// for the usual S x = 0 model that is the sampler's own set. Environment:
+// ACHR_SEED (random objectives), ACHR_WARMUP_ITERS (iteration cap, default
+// 40,000), ACHR_WARMUP_TOL (default 1e-9).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -40,12 +43,14 @@ int main(int argc, char** argv) {
   std::printf("\nThe model supplied is %s\n", argv[1]);
   const auto t0 = std::chrono::steady_clock::now();
   achr_model* model = nullptr;
-  if (achr_model_read_mps(argv[1], &model) != ACHR_OK) return die("reading the MPS model");
-  int n = 0, m = 0, nnz = 0, d = 0;
+  if (achr_model_read_mps_rows(argv[1], &model) != ACHR_OK) return die("reading the MPS model");
+  int n = 0, m = 0, nnz = 0, d = 0, nf = 0;
   achr_model_dims(model, &n, &m, &nnz, &d);
-  if (nPts < 2L * n) {
-    std::printf("Warning: nPts=%ld is less than 2*nRxns=%d, setting nPts=%d\n", nPts, 2 * n, 2 * n);
-    nPts = 2L * n;
+  achr_model_file_columns(model, &nf);
+  if (nf < n) std::printf("%d slack / right-hand-side columns carry the rows' types and right-hand sides\n", n - nf);
+  if (nPts < 2L * nf) {
+    std::printf("Warning: nPts=%ld is less than 2*nRxns=%d, setting nPts=%d\n", nPts, 2 * nf, 2 * nf);
+    nPts = 2L * nf;
   }
   if (nPts % 2) ++nPts;
   std::printf("Creating %ld warmup points! \n", nPts);
@@ -66,12 +71,12 @@ int main(int argc, char** argv) {
   const std::string fva_name = base + "_fva_bounds.csv";
   if (achr_write_fva_csv(fva_name.c_str(), model, fva.data()) != ACHR_OK) return die("writing the FVA bounds");
   std::printf("Raw FVA bounds written to %s\n", fva_name.c_str());
-  // one line per reaction, one column per point
-  std::vector<double> rows(W.size());
+  // one line per reaction of the file, one column per point
+  std::vector<double> rows(static_cast<size_t>(nf) * nPts);
   for (long b = 0; b < nPts; ++b)
-    for (int i = 0; i < n; ++i) rows[static_cast<size_t>(i) * nPts + b] = W[static_cast<size_t>(b) * n + i];
+    for (int i = 0; i < nf; ++i) rows[static_cast<size_t>(i) * nPts + b] = W[static_cast<size_t>(b) * n + i];
   const std::string name = base + std::to_string(nPts) + "warmup.csv";
-  if (achr_write_points_csv(name.c_str(), rows.data(), n, static_cast<int>(nPts)) != ACHR_OK)
+  if (achr_write_points_csv(name.c_str(), rows.data(), nf, static_cast<int>(nPts)) != ACHR_OK)
     return die("writing the warmup points");
   const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::printf("Warmup points written to %s in %.3f s\n", name.c_str(), t);

@@ -58,13 +58,39 @@ const char *achr_version(void);
 int achr_model_create(int n_rxns, int n_mets, int nnz, const int *rows, const int *cols,
                       const double *vals, const double *lb, const double *ub, achr_model **out);
 
-/* Model from an MPS file in the subset the reference reads through
- * glp_read_mps(GLP_MPS_DECK) (ACHRCuda.cu:560-597): N and E rows (the first N
- * row is the objective and is not part of S), COLUMNS with one or two entries
- * per line, RHS ignored, BOUNDS UP/LO/FX/FR/MI/PL (default bounds [0, +inf),
- * an infinite bound is stored as +-DBL_MAX as GLPK reports it). Free-format
- * whitespace splitting, so fixed-format decks read the same. */
+/* Model from an MPS file as the reference's sampler sees it through
+ * glp_read_mps(GLP_MPS_DECK) (ACHRCuda.cu:560-597): every row after the
+ * objective (the first N row) is a row of S whatever its type, RHS and RANGES
+ * are read and ignored (the sampled set is S x = 0), COLUMNS with one or two
+ * entries per line, BOUNDS UP/LO/FX/FR/MI/PL/BV/LI/UI (default bounds
+ * [0, +inf), an infinite bound is stored as +-DBL_MAX as GLPK reports it).
+ * Free-format whitespace splitting, so fixed-format decks read the same. */
 int achr_model_read_mps(const char *path, achr_model **out);
+
+/* The same file with its rows as written, the view GLPK gives the reference's
+ * warmup generator (createWarmupPtsGLPK.c:219): E rows S_i x = b_i, L rows
+ * S_i x <= b_i, G rows S_i x >= b_i, RANGES giving the other side by the MPS
+ * rules, free rows after the objective dropped. The model is returned as a
+ * set {x' : S' x' = 0, lb' <= x' <= ub'} whose first n_file columns are the
+ * file's and range over exactly its polytope: each row with two distinct
+ * sides gets a slack column (S_i x - s_i = 0, bounds the row's sides, an
+ * infinite side replaced by the row's activity range over the column bounds),
+ * and equality rows with b_i != 0 take -b_i on one last column fixed at 1.
+ * ACHR_E_FORMAT when a row is unbounded or unmeetable within the column
+ * bounds. No counterpart in the reference's sampler, which ignores row types
+ * and right-hand sides. */
+int achr_model_read_mps_rows(const char *path, achr_model **out);
+
+/* Number of leading columns that come from the file: n for every model but
+ * those of achr_model_read_mps_rows. */
+int achr_model_file_columns(const achr_model *model, int *n_file);
+
+/* Fills the auxiliary columns of P points given by their file columns
+ * (x_file: n_file doubles per point, points one after another; x: n doubles
+ * per point): a slack takes its row's activity, the right-hand-side column
+ * 1. achr_sampler_create_csv does this itself for a warmup file of n_file
+ * lines. */
+int achr_model_complete_points(const achr_model *model, const double *x_file, int P, double *x);
 
 void achr_model_destroy(achr_model *model);
 
@@ -349,7 +375,9 @@ void achr_warmup_options_default(achr_warmup_options *opts);
  * each solution clipped to the bounds and moved toward the points' mean,
  * x <- 0.33 x + 0.67 mean. warmup: n x nPts column major (each point
  * contiguous, the sampler's layout). fva (n x 2, may be NULL): the minimum and
- * maximum of each reaction. converged (nPts ints, may be NULL): 1 for each LP
+ * maximum of each reaction. For a model from achr_model_read_mps_rows the
+ * objectives range over its n_file file columns only (nPts >= 2 n_file, the
+ * reference's LPs) and fva holds NaN for the auxiliary columns. converged (nPts ints, may be NULL): 1 for each LP
  * whose iterate met tol, 0 for one stopped by max_iters (a first-order method:
  * its value is approximate, its point only approximately on S x = 0).
  * report (4 doubles, may be NULL): iterations, LPs converged, solve
@@ -360,8 +388,8 @@ int achr_warmup_generate(const achr_model *model, int nPts, const achr_warmup_op
 
 /* The reference generator's raw FVA file (createWarmupPtsGLPK.c:253-268):
  * "reaction,min,max" then one "name,%f,%f" line per reaction (names from the
- * MPS file; "?" for models built from arrays). fva: n x 2 as returned by
- * achr_warmup_generate. */
+ * MPS file; "?" for models built from arrays), the file's columns only.
+ * fva: n x 2 as returned by achr_warmup_generate. */
 int achr_write_fva_csv(const char *path, const achr_model *model, const double *fva);
 
 /* Lossless counterpart of achr_write_points_csv (whose "%f" keeps six
