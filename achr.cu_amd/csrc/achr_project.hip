@@ -5,12 +5,14 @@
 // (reprojectPoint / reprojectPoint2, ACHRcu/ACHRCuda.cu:208-230), which at
 // genome scale reads the whole of N (n x d doubles) per chain per check.
 // Here the flagged chains of a round are gathered and both products run as
-// register-tiled FP64 GEMMs, so N is read once per 128 chains. Each output is
-// accumulated with fused multiply-adds in increasing k, starting from +0.0 —
-// the loop order of the reference's two kernels — so the result is bitwise
-// the sequential dot product the host oracle computes. (On gfx950 FP64 vector
-// FMA has the same peak as FP64 MFMA, so nothing is lost by staying on the
-// VALU with a fixed accumulation order.)
+// tiled FP64 GEMMs on the matrix cores, so N is read once per 128 chains.
+// Each output is accumulated in increasing k starting from +0.0 — the loop
+// order of the reference's two kernels — so the result is bitwise the
+// sequential fused-multiply-add dot product the host oracle computes:
+// v_mfma_f64_16x16x4f64 adds its four products to the accumulator as four
+// chained fused multiply-adds in k order (checked bit for bit against that
+// chain, cancellations and wide exponent ranges included), and consecutive
+// instructions continue the chain through the accumulator.
 #include <algorithm>
 
 #include "achr_internal.h"
@@ -19,128 +21,162 @@
 namespace achr {
 namespace {
 
-constexpr int BM = 128, BN = 128, BK = 8, TM = 8, TN = 8;
-constexpr int kThreads = (BM / TM) * (BN / TN);  // 256
-constexpr int kLoads = (BK * BM) / kThreads;      // 4 elements of A and of B per thread and slice
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int kThreads = 256;                     // 4 wavefronts, each a 64 x 64 quarter of the tile
+constexpr int kPerThread = (BK * BM) / kThreads;  // 8 elements of A and of B per thread and slice
+constexpr int kRowStride = BK + 2;  // row-major tiles (k contiguous): 18 doubles per row
+constexpr int kColStride = BM + 16; // k-major tile of A: 144 doubles per k
+// With these strides both the fragment reads (16 rows x 4 k per wavefront)
+// and the tile stores spread over the LDS banks at the 2-way minimum that 64
+// lanes of 8 bytes need (3-way for the row-major stores).
 
 // C(r, q) = sum_k A(r, k) * B(k, q) for r < M, k < K, q < *count.
-// A(r, k) = A[r*sar + k*sak]; column q of B starts at B + colB(q)*ldb and is
-// contiguous in k; column q of C starts at Cm + colC(q)*ldc, contiguous in r.
-// colB(q) = idxB ? idxB[q] : q, likewise colC.
+// A(r, k) = A[r*sar + k*sak] (AK: sak == 1, rows contiguous in k); column q
+// of B starts at B + colB(q)*ldb and is contiguous in k; column q of C starts
+// at Cm + colC(q)*ldc, contiguous in r. colB(q) = idxB ? idxB[q] : q,
+// likewise colC.
 //
-// 128 x 128 output tile per 256-thread workgroup; thread (ty, tx) owns rows
-// ty*8 .. ty*8+7 and columns tx + 16 j (j < 8), 64 accumulators. Per k the
-// thread reads 8 + 8 doubles from LDS for 64 FMAs (2 bytes per FMA, what LDS
-// sustains at the FP64 FMA rate). The k loop runs over slices of 8 with two
-// LDS stages: the next slice is loaded into registers while the current one
-// is consumed, and stored into the other stage, one barrier per slice.
-// Padding past M, K or the column count reads 0.0, which adds nothing.
-__global__ __launch_bounds__(kThreads, 2) void dgemm_seq_kernel(int M, int K, const double* __restrict__ A,
-                                                              long sar, long sak, const double* __restrict__ B,
-                                                              long ldb, const int* __restrict__ idxB,
-                                                              double* __restrict__ Cm, long ldc,
-                                                              const int* __restrict__ idxC,
-                                                              const int* __restrict__ count) {
+// 128 x 128 output tile per 256-thread workgroup. Wavefront w owns rows
+// (w / 2) * 64 + [0, 64) and columns (w % 2) * 64 + [0, 64) as 4 x 4 blocks of
+// 16 x 16, each accumulated by v_mfma_f64_16x16x4f64 (lane l supplies A row
+// l % 16, k l / 16 and B k l / 16, column l % 16; it holds D rows
+// l / 16 + 4 i, column l % 16). Per 4-deep k step a lane reads 4 + 4 doubles
+// from LDS for 16 matrix instructions: 0.25 bytes per multiply-add. The k
+// loop runs over slices of 16 with two LDS stages: the next slice is loaded
+// into registers while the current one is consumed, and stored into the other
+// stage, one barrier per slice.
+//
+// Loads are unconditional: rows past M and columns past the count read row
+// M - 1 / column count - 1 (their outputs are never stored, and an output
+// depends only on its own row of A and column of B), and k past K reads
+// k = K - 1 and is replaced by 0.0, which adds nothing. Operands contiguous
+// in k are read 16 lanes per row, whole 128-byte lines per instruction.
+template <bool AK>
+__global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(int M, int K, const double* __restrict__ A,
+                                                               long sar, long sak, const double* __restrict__ B,
+                                                               long ldb, const int* __restrict__ idxB,
+                                                               double* __restrict__ Cm, long ldc,
+                                                               const int* __restrict__ idxC,
+                                                               const int* __restrict__ count) {
   const int ncols = *count;
   const int q0 = blockIdx.y * BN;
   if (q0 >= ncols) return;
   const int r0 = blockIdx.x * BM;
-  __shared__ double As[2][BK][BM + 4];  // padding 4: the k-strided stores spread over the banks
-  __shared__ double Bs[2][BK][BN + 4];
+  __shared__ double As[2][AK ? BM * kRowStride : BK * kColStride];
+  __shared__ double Bs[2][BN * kRowStride];
   const int t = threadIdx.x;
-  const int ty = t / (BN / TN), tx = t % (BN / TN);
 
-  // What this thread loads of each slice (4 elements of A, 4 of B), mapped so
-  // that each load instruction of a wavefront covers whole segments: operands
-  // contiguous in k take 8 lanes per row (64 bytes), A contiguous in rows
-  // takes 64 consecutive rows of one k.
-  const bool a_kcontig = (sak == 1);
-  int ar[kLoads], ak[kLoads];
+  // k-contiguous operands: lane k = t % 16 of rows / columns t / 16 + 16 e.
+  // A in general (AK false): 128 consecutive rows of k = t / 128 + 2 e.
+  const int lk = t & 15, lr = t >> 4;
+  const double* pA[AK ? kPerThread : 1];
+  long a_kstride = 0;
+  if constexpr (AK) {
 #pragma unroll
-  for (int e = 0; e < kLoads; ++e) {
-    if (a_kcontig) {
-      ar[e] = (t >> 3) + 32 * e;
-      ak[e] = t & 7;
-    } else {
-      ar[e] = t & 127;
-      ak[e] = (t >> 7) + 2 * e;
-    }
+    for (int e = 0; e < kPerThread; ++e)
+      pA[e] = A + static_cast<long>(min(r0 + lr + 16 * e, M - 1)) * sar;
+  } else {
+    pA[0] = A + static_cast<long>(min(r0 + (t & 127), M - 1)) * sar;
+    a_kstride = sak;
   }
-  const int bk = t & 7;
-  // Running pointers to this thread's elements of the current slice (null
-  // past M or past the column count), advanced by one slice per step.
-  const double* pA[kLoads];
-  const double* pB[kLoads];
+  const double* pB[kPerThread];
 #pragma unroll
-  for (int e = 0; e < kLoads; ++e) {
-    const int gr = r0 + ar[e];
-    pA[e] = gr < M ? A + static_cast<long>(gr) * sar + static_cast<long>(ak[e]) * sak : nullptr;
-    const int gq = q0 + (t >> 3) + 32 * e;
-    pB[e] = gq < ncols ? B + (idxB ? static_cast<long>(idxB[gq]) : static_cast<long>(gq)) * ldb + bk : nullptr;
+  for (int e = 0; e < kPerThread; ++e) {
+    const int q = min(q0 + lr + 16 * e, ncols - 1);
+    pB[e] = B + (idxB ? static_cast<long>(idxB[q]) : static_cast<long>(q)) * ldb;
   }
-  const long stepA = static_cast<long>(BK) * sak;
 
-  double pa[kLoads], pb[kLoads];
+  double pa[kPerThread], pb[kPerThread];
   auto load_slice = [&](int k0) {
+    const int kb = k0 + lk;
+    const long kc = min(kb, K - 1);
 #pragma unroll
-    for (int e = 0; e < kLoads; ++e) {
-      pa[e] = (pA[e] != nullptr && k0 + ak[e] < K) ? *pA[e] : 0.0;
-      pb[e] = (pB[e] != nullptr && k0 + bk < K) ? *pB[e] : 0.0;
-      if (pA[e] != nullptr) pA[e] += stepA;
-      if (pB[e] != nullptr) pB[e] += BK;
+    for (int e = 0; e < kPerThread; ++e) {
+      const double v = pB[e][kc];
+      pb[e] = kb < K ? v : 0.0;
+    }
+    if constexpr (AK) {
+#pragma unroll
+      for (int e = 0; e < kPerThread; ++e) {
+        const double v = pA[e][kc];
+        pa[e] = kb < K ? v : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < kPerThread; ++e) {
+        const int ka = k0 + (t >> 7) + 2 * e;
+        const double v = pA[0][static_cast<long>(min(ka, K - 1)) * a_kstride];
+        pa[e] = ka < K ? v : 0.0;
+      }
     }
   };
   auto store_slice = [&](int st) {
 #pragma unroll
-    for (int e = 0; e < kLoads; ++e) {
-      As[st][ak[e]][ar[e]] = pa[e];
-      Bs[st][bk][(t >> 3) + 32 * e] = pb[e];
+    for (int e = 0; e < kPerThread; ++e) {
+      if constexpr (AK)
+        As[st][(lr + 16 * e) * kRowStride + lk] = pa[e];
+      else
+        As[st][((t >> 7) + 2 * e) * kColStride + (t & 127)] = pa[e];
+      Bs[st][(lr + 16 * e) * kRowStride + lk] = pb[e];
     }
   };
 
-  double acc[TM][TN];
+  const int lane = t & 63, w = t >> 6;
+  const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+  const int fr = lane & 15, fk = lane >> 4;
+  f64x4 acc[4][4];
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = 0.0;
+    for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-  load_slice(0);
-  store_slice(0);
+  if (K > 0) {  // K == 0 (an empty null space): the outputs are zero
+    load_slice(0);
+    store_slice(0);
+  }
   __syncthreads();
   int st = 0;
   for (int k0 = 0; k0 < K; k0 += BK) {
     const bool more = k0 + BK < K;
     if (more) load_slice(k0 + BK);
-#pragma unroll 2
-    for (int k = 0; k < BK; ++k) {
-      double av[TM], bv[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) av[i] = As[st][k][ty * TM + i];
+    for (int kk = 0; kk < BK; kk += 4) {
+      double av[4], bv[4];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bv[j] = Bs[st][k][tx + 16 * j];
+      for (int i = 0; i < 4; ++i) {
+        if constexpr (AK)
+          av[i] = As[st][(wr + 16 * i + fr) * kRowStride + kk + fk];
+        else
+          av[i] = As[st][(kk + fk) * kColStride + wr + 16 * i + fr];
+      }
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[st][(wc + 16 * j + fr) * kRowStride + kk + fk];
 #pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
     }
     if (more) store_slice(st ^ 1);
     __syncthreads();
     st ^= 1;
   }
 #pragma unroll
-  for (int j = 0; j < TN; ++j) {
-    const int q = q0 + tx + 16 * j;
+  for (int j = 0; j < 4; ++j) {
+    const int q = q0 + wc + 16 * j + fr;
     if (q >= ncols) continue;
     double* ccol = Cm + (idxC ? static_cast<long>(idxC[q]) : static_cast<long>(q)) * ldc;
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
-      const int gr = r0 + ty * TM + i;
-      if (gr < M) ccol[gr] = acc[i][j];
-    }
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = r0 + wr + 16 * i + fk + 4 * e;
+        if (row < M) ccol[row] = acc[i][j][e];
+      }
   }
 }
 
-// Stable compaction of flag[0..P) into idx (increasing chain order); one block.
 __global__ __launch_bounds__(1024) void compact_kernel(const int* __restrict__ flag, int P, int* __restrict__ idx,
                                                         int* __restrict__ count) {
   __shared__ int wsum[16];
@@ -254,8 +290,14 @@ unsigned blocks_for(size_t n) {
 hipError_t launch_dgemm_cols(int M, int K, const double* A, long sar, long sak, const double* B, long ldb,
                              double* C, long ldc, const int* count, int max_cols, hipStream_t stream) {
   const unsigned gq = static_cast<unsigned>((max_cols + BN - 1) / BN);
-  hipLaunchKernelGGL(dgemm_seq_kernel, dim3((M + BM - 1) / BM, gq), dim3(kThreads), 0, stream, M, K, A, sar, sak, B,
-                     ldb, static_cast<const int*>(nullptr), C, ldc, static_cast<const int*>(nullptr), count);
+  if (M <= 0 || gq == 0) return hipSuccess;
+  if (sak == 1)
+    hipLaunchKernelGGL(dgemm_mfma_kernel<true>, dim3((M + BM - 1) / BM, gq), dim3(kThreads), 0, stream, M, K, A, sar,
+                       sak, B, ldb, static_cast<const int*>(nullptr), C, ldc, static_cast<const int*>(nullptr), count);
+  else
+    hipLaunchKernelGGL(dgemm_mfma_kernel<false>, dim3((M + BM - 1) / BM, gq), dim3(kThreads), 0, stream, M, K, A,
+                       sar, sak, B, ldb, static_cast<const int*>(nullptr), C, ldc, static_cast<const int*>(nullptr),
+                       count);
   return hipGetLastError();
 }
 
@@ -326,14 +368,17 @@ hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStrea
 hipError_t launch_project(const double* Ncm, int n, int d, double* X, double* T, const int* idx,
                           const int* count, int P, hipStream_t stream) {
   const unsigned gq = static_cast<unsigned>((P + BN - 1) / BN);
-  // T(:, q) = N^T X(:, idx[q]):   A(j, i) = N[i + j*n]  (sar = n, sak = 1)
-  hipLaunchKernelGGL(dgemm_seq_kernel, dim3((d + BM - 1) / BM, gq), dim3(kThreads), 0, stream, d, n, Ncm,
-                     static_cast<long>(n), 1L, X, static_cast<long>(n), idx, T, static_cast<long>(d),
-                     static_cast<const int*>(nullptr), count);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (gq == 0 || n <= 0) return hipSuccess;
+  // T(:, q) = N^T X(:, idx[q]):   A(j, i) = N[i + j*n]  (sar = n, sak = 1); nothing when d == 0
+  if (d > 0) {
+    hipLaunchKernelGGL(dgemm_mfma_kernel<true>, dim3((d + BM - 1) / BM, gq), dim3(kThreads), 0, stream, d, n, Ncm,
+                       static_cast<long>(n), 1L, X, static_cast<long>(n), idx, T, static_cast<long>(d),
+                       static_cast<const int*>(nullptr), count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   // X(:, idx[q]) = N T(:, q):     A(i, j) = N[i + j*n]  (sar = 1, sak = n)
-  hipLaunchKernelGGL(dgemm_seq_kernel, dim3((n + BM - 1) / BM, gq), dim3(kThreads), 0, stream, n, d, Ncm, 1L,
+  hipLaunchKernelGGL(dgemm_mfma_kernel<false>, dim3((n + BM - 1) / BM, gq), dim3(kThreads), 0, stream, n, d, Ncm, 1L,
                      static_cast<long>(n), T, static_cast<long>(d), static_cast<const int*>(nullptr), X,
                      static_cast<long>(n), idx, count);
   return hipGetLastError();
