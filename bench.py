@@ -215,7 +215,7 @@ def roofline_for(args_proj, m, n, d, P, S, q, tm, bank=False):
     step_ms, proj_ms = tm["ms_step"], tm["ms_proj"]
     if args_proj == "fp64":
         proj_flops = 4.0 * n * d * reproj
-        kname, peak, per = "dgemm_seq_kernel (batched N (N^T X), FP64 FMA)", PEAK_FP64_TFLOPS, "4*n*d"
+        kname, peak, per = "dgemm_mfma_kernel (batched N (N^T X), FP64 MFMA)", PEAK_FP64_TFLOPS, "4*n*d"
     else:
         proj_flops = reproj * (0.9 * 2.0 * n * m.m + 0.1 * 4.0 * n * d)
         kname = f"gemm_kernel ({args_proj}: X -= S+ (S X)) + FP64 rounds"
