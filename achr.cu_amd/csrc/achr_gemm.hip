@@ -38,7 +38,12 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
 
 // Loads the 128 x 32 tile of A (rows i0.., k0..) or of B^T (columns j0..,
 // k0..) as FP32 values vals[16] for this thread; position (r, k) of element
-// e is returned through rr/kk. `kcontig` picks the coalesced layout.
+// e is returned through rr/kk. `kcontig` picks the coalesced layout. All 16
+// loads are issued unconditionally, so they are in flight together: a row
+// past nrows reads row nrows - 1 and k past K reads K - 1, and those values
+// are then multiplied by 0 (operands are finite). A select instead of the
+// product lets the compiler skip the load under a branch and wait for each
+// load before the next. Needs nrows >= 1 and K >= 1.
 template <typename T>
 __device__ __forceinline__ void load_tile(const T* base, long s_row, long s_k, bool kcontig, const int* idx,
                                           int row0, int nrows, int k0, int K, float (&vals)[16], int& r_first,
@@ -52,16 +57,12 @@ __device__ __forceinline__ void load_tile(const T* base, long s_row, long s_k, b
     k_first = kb;
     along_k = true;
     const int gr = row0 + r;
-    if (gr < nrows) {
-      const long roff = static_cast<long>(idx ? idx[gr] : gr) * s_row;
+    const int grc = min(gr, nrows - 1);
+    const long roff = static_cast<long>(idx ? idx[grc] : grc) * s_row;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int gk = k0 + kb + e;
-        vals[e] = gk < K ? static_cast<float>(base[roff + gk]) : 0.0f;
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 16; ++e) vals[e] = 0.0f;
+    for (int e = 0; e < 16; ++e) {
+      const int gk = k0 + kb + e;
+      vals[e] = static_cast<float>(base[roff + min(gk, K - 1)]) * ((gr < nrows && gk < K) ? 1.0f : 0.0f);
     }
   } else {
     // thread -> one k, 16 consecutive rows (rows contiguous, no gather)
@@ -71,10 +72,12 @@ __device__ __forceinline__ void load_tile(const T* base, long s_row, long s_k, b
     k_first = kk;
     along_k = false;
     const int gk = k0 + kk;
+    const long koff = static_cast<long>(min(gk, K - 1)) * s_k;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int gr = row0 + rb + e;
-      vals[e] = (gr < nrows && gk < K) ? static_cast<float>(base[static_cast<long>(gr) + gk * s_k]) : 0.0f;
+      vals[e] = static_cast<float>(base[static_cast<long>(min(gr, nrows - 1)) + koff]) *
+                ((gr < nrows && gk < K) ? 1.0f : 0.0f);
     }
   }
 }
@@ -118,8 +121,10 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
   float va[16], vb[16];
   int ra, ka, rb, kb;
   bool ala, alb;
-  load_tile<TA>(A, g.sa_m, g.sa_k, a_kc, g.idxA, i0, M, 0, g.K, va, ra, ka, ala);
-  load_tile<float>(g.B, g.sb_n, g.sb_k, b_kc, nullptr, j0, g.N, 0, g.K, vb, rb, kb, alb);
+  if (g.K > 0) {
+    load_tile<TA>(A, g.sa_m, g.sa_k, a_kc, g.idxA, i0, M, 0, g.K, va, ra, ka, ala);
+    load_tile<float>(g.B, g.sb_n, g.sb_k, b_kc, nullptr, j0, g.N, 0, g.K, vb, rb, kb, alb);
+  }
   for (int k0 = 0; k0 < g.K; k0 += BK) {
     if constexpr (!kBF) {
       if (k0 > 0) {
@@ -206,19 +211,44 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g) {
     for (int b = 0; b < 2; ++b) {
       const int j = j0 + wc * 64 + b * 32 + (lane & 31);
       if (j >= g.N) continue;
+      // Four rows at a time: addresses (rows past M take row M - 1's, never
+      // stored), then the reads of C before any write, so they are in flight
+      // together without holding a whole column's worth of registers.
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int i = i0 + wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (i >= M) continue;
-        const long ci = static_cast<long>(g.idxC ? g.idxC[i] : i) * g.sc_m + static_cast<long>(j) * g.sc_n;
-        const float v = acc[a][b][r];
+      for (int r0 = 0; r0 < 16; r0 += 4) {
+        long ci[4];
+        bool ok[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = r0 + q;
+          const int i = i0 + wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          ok[q] = i < M;
+          const int ic = min(i, M - 1);
+          ci[q] = static_cast<long>(g.idxC ? g.idxC[ic] : ic) * g.sc_m + static_cast<long>(j) * g.sc_n;
+        }
         if (g.epi == kEpiStoreF32) {
           float* C = static_cast<float*>(g.C);
-          C[ci] = g.beta != 0.0f ? g.alpha * v + g.beta * C[ci] : g.alpha * v;
+          float old[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+          if (g.beta != 0.0f) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) old[q] = C[ci[q]];
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (ok[q]) C[ci[q]] = g.beta != 0.0f ? g.alpha * acc[a][b][r0 + q] + g.beta * old[q] : g.alpha * acc[a][b][r0 + q];
         } else if (g.epi == kEpiStoreF64) {
-          static_cast<double*>(g.C)[ci] = static_cast<double>(g.alpha * v);
+          double* C = static_cast<double*>(g.C);
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (ok[q]) C[ci[q]] = static_cast<double>(g.alpha * acc[a][b][r0 + q]);
         } else {
-          static_cast<double*>(g.C)[ci] -= static_cast<double>(g.alpha * v);
+          double* C = static_cast<double*>(g.C);
+          double old[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) old[q] = C[ci[q]];
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (ok[q]) C[ci[q]] = old[q] - static_cast<double>(g.alpha * acc[a][b][r0 + q]);
         }
       }
     }
