@@ -32,7 +32,12 @@ namespace achr {
 namespace {
 
 constexpr int kRedSlots = 16;  // wavefront partials per reduction (<= 1024 threads)
-constexpr int kPrefetchMaxE = 8;  // largest slot count that prefetches warmup points
+// Largest slot count that prefetches warmup points: at 11 slots (1,024
+// lanes, n = 10,600) the prefetch spills 4 VGPRs and takes 15% off a plain
+// step; 12 and more would spill the point. The batch kernel, which holds
+// its work-item bookkeeping as well, keeps the prefetch to 8 slots (at 10
+// and 11 it would spill 32 and 54 VGPRs).
+constexpr int kPrefetchMaxE = 11, kPrefetchMaxEBatch = 8;
 // LDS slot arrays of the block-per-chain reductions: the norm, two pairs for
 // min/max and the residual maximum. Consecutive reductions must use disjoint
 // arrays (a wavefront may write the next partial while a slower one still
@@ -231,7 +236,7 @@ struct Row {
 // the block is one chain, so every early return is block-uniform). The
 // arguments come by value: a reference to the kernel's argument block keeps
 // the compiler from scalarising it and costs about 40 more spilled VGPRs.
-template <int TPC, int E, int CPB, bool DIR>
+template <int TPC, int E, int CPB, bool DIR, bool BATCH>
 __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, double* lds) {
   // With 64 or more lanes the chain owns whole wavefronts: its index is
   // wavefront-uniform, and so is everything it branches on.
@@ -318,9 +323,9 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
 
   // The warmup point (or bank direction) of attempt a + 1 is loaded while
   // attempt a finishes: the pick is a function of the attempt counter only.
-  // Without a bank, schedules of more than 8 slots load the point at the
-  // top of the step instead (the prefetch registers would spill).
-  constexpr bool kPre = DIR || E <= kPrefetchMaxE;
+  // Without a bank, schedules of more than kPrefetchMaxE slots load the
+  // point at the top of the step instead (the prefetch registers would spill).
+  constexpr bool kPre = DIR || E <= (BATCH ? kPrefetchMaxEBatch : kPrefetchMaxE);
   double wn[kPre ? E : 1];
   auto load_dir = [&](uint32_t att) {
     if constexpr (!kPre) return;
@@ -584,7 +589,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
 template <int TPC, int E, int CPB, bool DIR>
 __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
   extern __shared__ double lds[];
-  run_chain<TPC, E, CPB, DIR>(a, static_cast<int>(blockIdx.x) * CPB + static_cast<int>(threadIdx.x) / TPC, lds);
+  run_chain<TPC, E, CPB, DIR, false>(a, static_cast<int>(blockIdx.x) * CPB + static_cast<int>(threadIdx.x) / TPC, lds);
 }
 
 // Several models in one persistent launch: the grid is sized to what stays
@@ -602,7 +607,7 @@ __global__ __launch_bounds__(TPC* CPB) void step_batch_kernel(const StepArgs* __
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int2 w = items[it];
     const StepArgs a = tab[w.x];
-    run_chain<TPC, E, CPB, DIR>(a, w.y + static_cast<int>(threadIdx.x) / TPC, lds);
+    run_chain<TPC, E, CPB, DIR, true>(a, w.y + static_cast<int>(threadIdx.x) / TPC, lds);
     if constexpr (TPC > 64) __syncthreads();
   }
 }

@@ -1,8 +1,9 @@
 """Genome-scale chains of 1,024 lanes keep the per-reaction bounds on chip:
 the first 9 slot rows in LDS as (lb, ub) pairs, up to 2 more rows in
 registers, any further row read per step. Bank mode at 10, 11 and 12 slots
-(so every kind of storage is exercised) and plain ACHR at 8 slots (all in
-LDS): GPU and oracle agree bit for bit, points, counters and traces. The bank
+(so every kind of storage is exercised), plain ACHR at 8 slots (all in
+LDS) and at 9 and 11 slots (centre in LDS, next warmup point prefetched):
+GPU and oracle agree bit for bit, points, counters and traces. The bank
 mode's reciprocal (hardware estimate + three Newton steps) must equal IEEE
 division on the whole range the chord uses."""
 import numpy as np
@@ -75,6 +76,22 @@ def test_plain_chain_with_bounds_in_lds_bitwise(mets, rxns, lanes):
     o, ost, _ = oracle_run(m, W, Sd, bank=None, lanes=lanes, P=P, S=S, seed=seed, trace=0, thin=0)
     np.testing.assert_array_equal(smp.stats(), ost)
     np.testing.assert_array_equal(smp.points(), o)
+
+
+@pytest.mark.parametrize("mets,rxns", [(5000, 9200), (5800, 10600)])
+def test_plain_chain_with_prefetch_and_centre_in_lds_bitwise(mets, rxns):
+    """Plain ACHR at 9 and 11 slots of 1,024 lanes: the next warmup point is
+    prefetched into registers while the running centre lives in LDS."""
+    say(f"plain prefetch case {mets} x {rxns}")
+    m, W, model, Sd = genome_case(mets, rxns, 41)
+    smp = achr.Sampler(model, W, achr.Options(lanes=1024, reproject_mode=2, trace_chains=2, trace_thin=5))
+    assert smp.schedule() == (1024, 2)
+    P, S, seed = 6, 25, 6
+    smp.run(P, S, seed=seed)
+    o, ost, otr = oracle_run(m, W, Sd, bank=None, lanes=1024, P=P, S=S, seed=seed, trace=2, thin=5)
+    np.testing.assert_array_equal(smp.stats(), ost)
+    np.testing.assert_array_equal(smp.points(), o)
+    np.testing.assert_array_equal(smp.trace(), otr)
 
 
 def test_bank_reciprocal_is_ieee_division():
