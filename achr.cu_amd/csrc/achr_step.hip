@@ -38,6 +38,7 @@ constexpr int kRedSlots = 16;  // wavefront partials per reduction (<= 1024 thre
 // its work-item bookkeeping as well, keeps the prefetch to 8 slots (at 10
 // and 11 it would spill 32 and 54 VGPRs).
 constexpr int kPrefetchMaxE = 11, kPrefetchMaxEBatch = 8;
+constexpr int kBoundLookahead = 2;  // slots between a bound load and its use
 // LDS slot arrays of the block-per-chain reductions: the norm, two pairs for
 // min/max and the residual maximum. Consecutive reductions must use disjoint
 // arrays (a wavefront may write the next partial while a slower one still
@@ -397,6 +398,15 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
           }
         }
       }
+      // Bounds read from memory (slots past the LDS-resident rows) are
+      // requested kLook slots ahead of their use (the first ones before the
+      // norm's reduction), so their L2 latency overlaps the divisions of the
+      // slots before. The batch kernel, short of registers, reads them at use.
+      constexpr int kLook = BATCH ? 0 : kBoundLookahead;
+      double2 bq[E];
+#pragma unroll
+      for (int k = 0; k < (E < kLook ? E : kLook); ++k)
+        if (k >= BP::lds) bq[k] = bounds(k);
       // Bank directions are unit vectors: no norm, no division.
       double nrm = 1.0;
       if constexpr (!DIR) {
@@ -409,7 +419,8 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
       for (int k = 0; k < E; ++k) {
         const double ui = DIR ? u[k] : u[k] / nrm;
         u[k] = ui;
-        const double2 bk = bounds(k);
+        if (kLook > 0 && k + kLook < E && k + kLook >= BP::lds) bq[k + kLook] = bounds(k + kLook);
+        const double2 bk = (kLook > 0 && k >= BP::lds) ? bq[k] : bounds(k);
         const double lbi = bk.x, ubi = bk.y;
         if constexpr (DIR) {
           // the clip of the last accepted step, deferred to here where the
