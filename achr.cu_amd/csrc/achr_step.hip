@@ -128,14 +128,48 @@ __device__ __forceinline__ double vmin(double a, double b) {
   return r;
 }
 
+// A double moved between lanes by a DPP pattern (VALU only, no LDS trip).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false));
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), lane));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane));
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+// Maximum (MAX) or minimum of a wavefront's 64 values: quad exchanges, the
+// half-row and row mirrors leave every lane of a 16-lane row with the row's
+// extremum, and the four rows are combined from lanes 0, 16, 32, 48. The
+// extremum of a set does not depend on the pairing, so this equals the
+// butterfly exactly.
+template <bool MAX>
+__device__ __forceinline__ double wave_extremum(double v) {
+  auto op = [](double a, double b) { return MAX ? vmax(a, b) : vmin(a, b); };
+  v = op(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = op(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = op(v, dpp_f64<0x141>(v));  // row_half_mirror
+  v = op(v, dpp_f64<0x140>(v));  // row_mirror
+  return op(op(readlane_f64(v, 0), readlane_f64(v, 16)), op(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
 // Maxima and minima of values that are never NaN over the chain's lanes.
 template <int TPC>
 __device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red_mn, double* red_mx) {
   constexpr int G = TPC < 64 ? TPC : 64;
+  if constexpr (G == 64) {
+    mn = wave_extremum<true>(mn);
+    mx = wave_extremum<false>(mx);
+  } else {
 #pragma unroll
-  for (int o = G / 2; o >= 1; o >>= 1) {
-    mn = vmax(mn, __shfl_xor(mn, o));
-    mx = vmin(mx, __shfl_xor(mx, o));
+    for (int o = G / 2; o >= 1; o >>= 1) {
+      mn = vmax(mn, __shfl_xor(mn, o));
+      mx = vmin(mx, __shfl_xor(mx, o));
+    }
   }
   if constexpr (TPC > 64) {
     constexpr int NW = TPC / 64;
@@ -155,8 +189,12 @@ __device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red
 template <int TPC>
 __device__ __forceinline__ double group_max(double v, double* red) {
   constexpr int G = TPC < 64 ? TPC : 64;
+  if constexpr (G == 64) {
+    v = wave_extremum<true>(v);
+  } else {
 #pragma unroll
-  for (int o = G / 2; o >= 1; o >>= 1) v = vmax(v, __shfl_xor(v, o));
+    for (int o = G / 2; o >= 1; o >>= 1) v = vmax(v, __shfl_xor(v, o));
+  }
   if constexpr (TPC > 64) {
     constexpr int NW = TPC / 64;
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
