@@ -148,12 +148,17 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // extremum of a set does not depend on the pairing, so this equals the
 // butterfly exactly.
 template <bool MAX>
-__device__ __forceinline__ double wave_extremum(double v) {
+__device__ __forceinline__ double row_extremum(double v) {
   auto op = [](double a, double b) { return MAX ? vmax(a, b) : vmin(a, b); };
   v = op(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
   v = op(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
   v = op(v, dpp_f64<0x141>(v));  // row_half_mirror
-  v = op(v, dpp_f64<0x140>(v));  // row_mirror
+  return op(v, dpp_f64<0x140>(v));  // row_mirror
+}
+template <bool MAX>
+__device__ __forceinline__ double wave_extremum(double v) {
+  auto op = [](double a, double b) { return MAX ? vmax(a, b) : vmin(a, b); };
+  v = row_extremum<MAX>(v);
   return op(op(readlane_f64(v, 0), readlane_f64(v, 16)), op(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
@@ -178,11 +183,12 @@ __device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red
       red_mx[threadIdx.x >> 6] = mx;
     }
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NW; ++i) {
-      mn = vmax(mn, red_mn[i]);
-      mx = vmin(mx, red_mx[i]);
-    }
+    // lane i of every wavefront takes wavefront i's partial (lanes past NW
+    // repeat partial 0) and the first row reduces them
+    const int l = static_cast<int>(threadIdx.x & 63);
+    const int src = l < NW ? l : 0;
+    mn = readlane_f64(row_extremum<true>(red_mn[src]), 0);
+    mx = readlane_f64(row_extremum<false>(red_mx[src]), 0);
   }
 }
 
@@ -199,8 +205,8 @@ __device__ __forceinline__ double group_max(double v, double* red) {
     constexpr int NW = TPC / 64;
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NW; ++i) v = vmax(v, red[i]);
+    const int l = static_cast<int>(threadIdx.x & 63);
+    v = readlane_f64(row_extremum<true>(red[l < NW ? l : 0]), 0);
   }
   return v;
 }
