@@ -90,6 +90,20 @@ __device__ __forceinline__ void group_sync() {
   }
 }
 
+// A double moved between lanes by a DPP pattern (VALU only, no LDS trip).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false));
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), lane));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane));
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
 template <int TPC>
 __device__ __forceinline__ double group_sum(double v, double* red) {
   constexpr int G = TPC < 64 ? TPC : 64;
@@ -128,20 +142,6 @@ __device__ __forceinline__ double vmin(double a, double b) {
   return r;
 }
 
-// A double moved between lanes by a DPP pattern (VALU only, no LDS trip).
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false));
-  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false));
-  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
-}
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), lane));
-  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane));
-  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
-}
 // Maximum (MAX) or minimum of a wavefront's 64 values: quad exchanges, the
 // half-row and row mirrors leave every lane of a 16-lane row with the row's
 // extremum, and the four rows are combined from lanes 0, 16, 32, 48. The
