@@ -55,37 +55,44 @@ __device__ __forceinline__ double objective(const Lp& p, int i, int b) {
   return sgn * p.dc[i] * p.Cr[static_cast<size_t>(i) * R + (b - 2 * p.nfva)];
 }
 
-// x <- clamp(x - tau (c - A^T y)); xbar = 2 x_new - x; xsum += x_new
+// x <- clamp(x - tau (c - A^T y)); xbar = 2 x_new - x; xsum += x_new.
+// Grid: LPs across x, columns across y (strided). The column's CSC entries,
+// scaling and bounds depend on the block only, so they are read once per
+// wavefront through the scalar unit; the iterate reads are coalesced over b.
 __global__ __launch_bounds__(256) void primal_kernel(Lp p, double* __restrict__ X, double* __restrict__ Xbar,
                                                      double* __restrict__ Xsum, const double* __restrict__ Y,
                                                      const double* __restrict__ tau, const int* __restrict__ done) {
-  const size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (idx >= static_cast<size_t>(p.n) * p.B) return;
-  const int i = static_cast<int>(idx / p.B), b = static_cast<int>(idx % p.B);
-  if (done[b]) return;
-  double g = 0.0;
-  for (int q = p.cp[i]; q < p.cp[i + 1]; ++q) g = fma(p.vc[q], Y[static_cast<size_t>(p.ri[q]) * p.B + b], g);
-  const double x = X[idx];
-  double xn = x - tau[b] * (objective(p, i, b) - g);
-  xn = fmin(fmax(xn, p.l[i]), p.u[i]);
-  X[idx] = xn;
-  Xbar[idx] = 2.0 * xn - x;
-  Xsum[idx] += xn;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= p.B || done[b]) return;
+  const double tb = tau[b];
+  for (int i = blockIdx.y; i < p.n; i += gridDim.y) {
+    const size_t idx = static_cast<size_t>(i) * p.B + b;
+    double g = 0.0;
+    for (int q = p.cp[i]; q < p.cp[i + 1]; ++q) g = fma(p.vc[q], Y[static_cast<size_t>(p.ri[q]) * p.B + b], g);
+    const double x = X[idx];
+    double xn = x - tb * (objective(p, i, b) - g);
+    xn = fmin(fmax(xn, p.l[i]), p.u[i]);
+    X[idx] = xn;
+    Xbar[idx] = 2.0 * xn - x;
+    Xsum[idx] += xn;
+  }
 }
 
-// y <- y - sigma A xbar; ysum += y_new
+// y <- y - sigma A xbar; ysum += y_new (grid as in primal_kernel, rows across y)
 __global__ __launch_bounds__(256) void dual_kernel(Lp p, double* __restrict__ Y, double* __restrict__ Ysum,
                                                    const double* __restrict__ Xbar, const double* __restrict__ sig,
                                                    const int* __restrict__ done) {
-  const size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (idx >= static_cast<size_t>(p.m) * p.B) return;
-  const int j = static_cast<int>(idx / p.B), b = static_cast<int>(idx % p.B);
-  if (done[b]) return;
-  double r = 0.0;
-  for (int q = p.rp[j]; q < p.rp[j + 1]; ++q) r = fma(p.va[q], Xbar[static_cast<size_t>(p.ci[q]) * p.B + b], r);
-  const double y = Y[idx] - sig[b] * r;
-  Y[idx] = y;
-  Ysum[idx] += y;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= p.B || done[b]) return;
+  const double sb = sig[b];
+  for (int j = blockIdx.y; j < p.m; j += gridDim.y) {
+    const size_t idx = static_cast<size_t>(j) * p.B + b;
+    double r = 0.0;
+    for (int q = p.rp[j]; q < p.rp[j + 1]; ++q) r = fma(p.va[q], Xbar[static_cast<size_t>(p.ci[q]) * p.B + b], r);
+    const double y = Y[idx] - sb * r;
+    Y[idx] = y;
+    Ysum[idx] += y;
+  }
 }
 
 // Partial sums over row chunks: |A x|^2 and |y - y_r|^2 for the current and
@@ -435,6 +442,9 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
   Lp p{n, m, B, nf, d_rp.p, d_ci.p, d_va.p, d_cp.p, d_ri.p, d_vc.p, d_l.p, d_u.p, d_dc.p, d_Cr.p};
   ColState cs{w.p, tau.p, sig.p, kr.p, kprev.p, den.p, d_since.p, d_done.p, d_action.p};
   const dim3 kb(256), rg(blocks_for(B), rchunks), cg(blocks_for(B), cchunks);
+  // iteration kernels: LPs across x, columns / rows across y (grid-strided past 65,535)
+  const unsigned bx = static_cast<unsigned>((B + 255) / 256);
+  const dim3 pg(bx, static_cast<unsigned>(std::min(n, 65535))), dg(bx, static_cast<unsigned>(std::max(std::min(m, 65535), 1)));
   auto kkt = [&]() {
     hipLaunchKernelGGL(kkt_rows_kernel, rg, kb, 0, nullptr, p, X.p, Xsum.p, Y.p, Ysum.p, Yr.p, den.p, prow.p);
     hipLaunchKernelGGL(kkt_cols_kernel, cg, kb, 0, nullptr, p, X.p, Xsum.p, Xr.p, Y.p, Ysum.p, den.p, pcol.p);
@@ -449,10 +459,8 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
   while (it < o.max_iters) {
     const int k = static_cast<int>(std::min<long long>(o.check_every, o.max_iters - it));
     for (int s = 0; s < k; ++s) {
-      hipLaunchKernelGGL(primal_kernel, dim3(blocks_for(nB)), kb, 0, nullptr, p, X.p, Xbar.p, Xsum.p, Y.p, tau.p,
-                         d_done.p);
-      if (m > 0)
-        hipLaunchKernelGGL(dual_kernel, dim3(blocks_for(mB)), kb, 0, nullptr, p, Y.p, Ysum.p, Xbar.p, sig.p, d_done.p);
+      hipLaunchKernelGGL(primal_kernel, pg, kb, 0, nullptr, p, X.p, Xbar.p, Xsum.p, Y.p, tau.p, d_done.p);
+      if (m > 0) hipLaunchKernelGGL(dual_kernel, dg, kb, 0, nullptr, p, Y.p, Ysum.p, Xbar.p, sig.p, d_done.p);
     }
     it += k;
     // the averages hold den + k iterates now
