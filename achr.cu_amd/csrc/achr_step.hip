@@ -549,8 +549,25 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         float* rrow = (a.mode == 2 && a.R32) ? a.R32 + static_cast<size_t>(b) * a.m : nullptr;
         for (int j = lane; j < a.m; j += TPC) {
           double acc = 0.0;
-          const int p1 = a.rowptr[j + 1];
-          for (int p = a.rowptr[j]; p < p1; ++p) acc = fma(a.vals[p], xrow.ld(a.colidx[p]), acc);
+          const int p0 = a.rowptr[j], p1 = a.rowptr[j + 1];
+          // four nonzeros at a time: their index, value and x loads are in
+          // flight together (positions past the row read its last entry and
+          // are skipped), then the products are added in order
+          for (int p = p0; p < p1; p += 4) {
+            int ci[4];
+            double vv[4], xv[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int q = min(p + e, p1 - 1);
+              ci[e] = a.colidx[q];
+              vv[e] = a.vals[q];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) xv[e] = xrow.ld(ci[e]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (p + e < p1) acc = fma(vv[e], xv[e], acc);
+          }
           const double av = fabs(acc);
           if (av > rho) rho = av;
           if (rrow) rrow[j] = static_cast<float>(acc);
