@@ -39,7 +39,7 @@ PROJ = {"fp64": 0, "res_bf16x3": 1, "res_f32": 2, "null_f32": 3, "null_bf16x3": 
 # rocprofv3 FETCH_SIZE / WRITE_SIZE and calibrated on a 1 GiB device copy
 # (profiles/step_kernel_bank_genome_pmc_traffic.md), keyed by
 # (reactions, chains, steps per launch); other shapes report null.
-PMC_TRAFFIC = {(10600, 20000, 100): (178.1e9, 173.0e9, "profiles/step_kernel_bank_genome_pmc_traffic.md")}
+PMC_TRAFFIC = {(10600, 20000, 100): (179.4e9, 173.0e9, "profiles/step_kernel_bank_genome_pmc_traffic.md")}
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
