@@ -397,6 +397,9 @@ int achr_model_read_mps_rows(const char* path, achr_model** out) {
     }
     row_new[i] = m++;
     if (lo == hi) {
+      if (lo < act_lo[i] || lo > act_hi[i])
+        return fail(ACHR_E_FORMAT, "%s: row %s cannot be met within the column bounds ([%g, %g] excludes %g)", path,
+                    F.row_names[i].c_str(), act_lo[i], act_hi[i], lo);
       if (b != 0.0) {
         rhs_coef[i] = -b;
         need_one = true;

@@ -175,6 +175,7 @@ def test_rhs_column_and_completion(tmp_path):
 @pytest.mark.parametrize("rows,lb,ub,why", [
     ([("L", 1.0, None)], [-BIG, 0.0], [1.0, 1.0], "unbounded below"),  # x0 + x1 <= 1 with x0 free below
     ([("G", 5.0, None)], [0.0, 0.0], [1.0, 1.0], "cannot be met"),     # x0 + x1 >= 5 in the unit square
+    ([("E", 3.0, None)], [0.0, 0.0], [1.0, 1.0], "cannot be met"),     # x0 + x1 = 3 in the unit square
 ])
 def test_rows_view_errors(tmp_path, rows, lb, ub, why):
     p = tmp_path / "bad.mps"
