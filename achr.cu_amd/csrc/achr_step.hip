@@ -107,8 +107,23 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 template <int TPC>
 __device__ __forceinline__ double group_sum(double v, double* red) {
   constexpr int G = TPC < 64 ? TPC : 64;
+  if constexpr (G == 64) {
+    // The butterfly's stages 32 and 16 as shuffles; stages 8, 4, 2 and 1 by
+    // row_shl:o, which hands lane i the value of lane i + o = i ^ o for the
+    // lanes below o in each row. Lane 0 depends only on such lanes, so it
+    // forms exactly the butterfly's sums, in the same operand order; it is
+    // the lane whose value is used (the wavefront's partial, or broadcast).
+    v = v + __shfl_xor(v, 32);
+    v = v + __shfl_xor(v, 16);
+    v = v + dpp_f64<0x108>(v);
+    v = v + dpp_f64<0x104>(v);
+    v = v + dpp_f64<0x102>(v);
+    v = v + dpp_f64<0x101>(v);
+    if constexpr (TPC == 64) v = readlane_f64(v, 0);
+  } else {
 #pragma unroll
-  for (int o = G / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o);
+    for (int o = G / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o);
+  }
   if constexpr (TPC > 64) {
     constexpr int NW = TPC / 64;
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
