@@ -184,6 +184,15 @@ __device__ __forceinline__ void group_minmax(double& mn, double& mx, double* red
   if constexpr (G == 64) {
     mn = wave_extremum<true>(mn);
     mx = wave_extremum<false>(mx);
+  } else if constexpr (G >= 16) {
+    // groups of 16 or 32 lanes: each 16-lane row by DPP, then the two rows
+    // of a 32-lane group exchanged once
+    mn = row_extremum<true>(mn);
+    mx = row_extremum<false>(mx);
+    if constexpr (G == 32) {
+      mn = vmax(mn, __shfl_xor(mn, 16));
+      mx = vmin(mx, __shfl_xor(mx, 16));
+    }
   } else {
 #pragma unroll
     for (int o = G / 2; o >= 1; o >>= 1) {
