@@ -157,11 +157,9 @@ __device__ __forceinline__ double vmin(double a, double b) {
   return r;
 }
 
-// Maximum (MAX) or minimum of a wavefront's 64 values: quad exchanges, the
-// half-row and row mirrors leave every lane of a 16-lane row with the row's
-// extremum, and the four rows are combined from lanes 0, 16, 32, 48. The
-// extremum of a set does not depend on the pairing, so this equals the
-// butterfly exactly.
+// Maximum (MAX) or minimum over each 16-lane row, left in every lane of the
+// row: quad exchanges, then the half-row and row mirrors. The extremum of a
+// set does not depend on the pairing, so this equals a butterfly exactly.
 template <bool MAX>
 __device__ __forceinline__ double row_extremum(double v) {
   auto op = [](double a, double b) { return MAX ? vmax(a, b) : vmin(a, b); };
@@ -170,6 +168,8 @@ __device__ __forceinline__ double row_extremum(double v) {
   v = op(v, dpp_f64<0x141>(v));  // row_half_mirror
   return op(v, dpp_f64<0x140>(v));  // row_mirror
 }
+// The same over a wavefront's 64 values: the four row results are combined
+// from lanes 0, 16, 32 and 48 (uniform over the wavefront).
 template <bool MAX>
 __device__ __forceinline__ double wave_extremum(double v) {
   auto op = [](double a, double b) { return MAX ? vmax(a, b) : vmin(a, b); };
