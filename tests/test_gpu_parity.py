@@ -65,6 +65,27 @@ def test_batched_projection_bitwise(c1_setup, lanes):
     np.testing.assert_array_equal(g, o)
 
 
+@pytest.mark.parametrize("lanes,mode", [(32, 1), (1024, 1), (256, 2)])
+def test_edge_sizes_one_chain_ragged_and_zero_steps(c1_setup, lanes, mode):
+    """One chain, chain counts that fill no whole block, and zero steps. The
+    reference starts each chain at the warmup centre (ACHRCuda.cu:374-377) and
+    writes its point only inside the step loop (:349), so with stepsPerPoint 0
+    its File<k> holds whatever the buffer held; here a chain with no steps
+    returns its start, the centre. A run of no chains is refused."""
+    for P, S in [(1, 150), (3, 57), (5, 0)]:
+        g, gs, _, o, os_, _ = run_both(c1_setup, c1_setup["Wf"], lanes=lanes, mode=mode, P=P, S=S, seed=31)
+        np.testing.assert_array_equal(gs, os_)
+        np.testing.assert_array_equal(g, o)
+        assert g.shape == (95, P)
+    smp = achr.Sampler(c1_setup["model"], c1_setup["Wf"], achr.Options(lanes=lanes, reproject_mode=mode))
+    np.testing.assert_array_equal(g, np.repeat(smp.center()[:, None], 5, axis=1))
+    assert (gs == 0).all()
+    for P, S in [(0, 10), (-1, 10), (4, -1)]:
+        with pytest.raises(achr.AchrError) as e:
+            smp.run(P, S, seed=1)
+        assert e.value.code == -1
+
+
 def test_trace_file_and_chain_offset(c1_setup):
     g, gs, gt, o, os_, ot = run_both(c1_setup, c1_setup["Wf"], lanes=64, mode=1, P=40, S=100, seed=9, file=3,
                                      chain0=1000, trace=5, thin=10)
