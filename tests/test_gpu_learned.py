@@ -110,6 +110,35 @@ def test_bank_mode_bitwise_against_oracle(c1):
         np.testing.assert_array_equal(smp.trace(), otr)  # recorded points are the clipped ones
 
 
+def test_bank_mode_edge_sizes(c1):
+    """Bank sampler with one chain, chain counts that fill no whole block, and
+    zero steps (each chain returns its start point as it is), bit-equal to the
+    oracle."""
+    import oracle as O
+    S, lb, ub, Wf, We = c1
+    N = O.nullspace(S)
+    rng = np.random.default_rng(8)
+    free = synth.free_directions(N, lb, ub)
+    bank = free @ rng.standard_normal((free.shape[1], 40))
+    bank /= np.linalg.norm(bank, axis=0)
+    model = achr.Model.from_dense(S, lb, ub)
+    model.set_nullspace(N)
+    rp, ci, va = O.csr_of(S)
+    for lanes, mode in ((32, 1), (1024, 2)):
+        smp = achr.Sampler(model, Wf, achr.Options(lanes=lanes, reproject_mode=mode))
+        smp.set_directions(bank)
+        smp.use_directions(True)
+        s0 = smp.start_point()
+        for P, Ss in ((1, 150), (3, 57), (5, 0)):
+            print(f"bank edge lanes={lanes} mode={mode} P={P} S={Ss}", flush=True)
+            g = smp.sample(P, Ss, seed=21)
+            o, ost, _ = O.sample(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S.shape[0], P=P,
+                                 S=Ss, seed=21, lanes=lanes, bank=bank, c0=s0)
+            np.testing.assert_array_equal(smp.stats(), ost)
+            np.testing.assert_array_equal(g, o)
+        np.testing.assert_array_equal(g, np.repeat(s0[:, None], 5, axis=1))
+
+
 def test_bank_chains_start_on_the_flux_set_and_never_reproject():
     """The centre of a "%f" warmup is off S x = 0 by the file's rounding. Bank
     chains start from the nearest point with S x = 0 and every fixed reaction
