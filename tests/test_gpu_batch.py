@@ -92,6 +92,28 @@ def test_batch_matches_oracle_and_solo_runs(cases):
         np.testing.assert_array_equal(s.stats(), got[k][1])
 
 
+def test_batch_edge_sizes(cases):
+    """A batch of one to three chains per model, then zero steps (every chain
+    returns the warmup centre), in the graph: each model bit-equal to the
+    oracle. A model with no chains is refused."""
+    smps = samplers_for(cases)
+    batch = achr.Batch(smps)
+    lanes = batch.lanes()
+    P = [1, 3, 1, 2, 1]
+    for S in (57, 0):
+        say(f"batch edge P={P} S={S}")
+        batch.run(P, S, seed=13, file=1)
+        for k, c in enumerate(cases):
+            o_pts, o_st = oracle_of(c, P=P[k], S=S, seed=13, file=1, model_id=k + 1, lanes=int(lanes[k]))
+            np.testing.assert_array_equal(smps[k].stats(), o_st)
+            np.testing.assert_array_equal(smps[k].points(), o_pts)
+    for k, s in enumerate(smps):
+        np.testing.assert_array_equal(s.points(), np.repeat(s.center()[:, None], P[k], axis=1))
+    with pytest.raises(achr.AchrError) as e:
+        batch.run([1, 0, 1, 1, 1], 10, seed=1)
+    assert e.value.code == -1
+
+
 def test_autotune_then_pinned_lanes_reproduce(cases):
     P = [64, 96, 64, 40, 24]
     smps = samplers_for(cases)
