@@ -183,3 +183,77 @@ def test_miss_cap_flags_degenerate_polytope():
     pts, st, _ = O.sample(**_box_model(lb, ub, Wm), P=2, S=5, seed=0, lanes=16, max_misses=50)
     assert (st[:, 3] == 1).all() and (st[:, 0] == 51).all()
     np.testing.assert_array_equal(pts, 0.0)
+
+
+def _chord(x, u, lb, ub):
+    """Chord [tmin, tmax] of fillrandPoint (ACHRCuda.cu:266-288) in numpy."""
+    du, dl = ub - x, x - lb
+    keep = (du > 1e-14) & (dl > 1e-14)
+    pos, neg = keep & (u > 1e-9), keep & (u < -1e-9)
+    tmin = max(np.max(-dl[pos] / u[pos], initial=-np.inf), np.max(du[neg] / u[neg], initial=-np.inf))
+    tmax = min(np.min(du[pos] / u[pos], initial=np.inf), np.min(-dl[neg] / u[neg], initial=np.inf))
+    return tmin, tmax
+
+
+@pytest.mark.parametrize("step", [0, 10, 20])
+def test_reprojection_value_replayed_with_scipy_projector(c1, step):
+    """The value of the reprojection x <- N (N^T x) (reprojectPoint /
+    reprojectPoint2, ACHRCuda.cu:208-230, triggered at :325-343) on the
+    "%f" warmup, where every check reprojects: step `step` (a check step) is
+    replayed in numpy from the recorded trajectory as
+    clip(P (x + t u)) with P the projector of scipy's null_space basis, not
+    the oracle's N. A transposed N, a dropped null-space column or a skipped
+    projection moves the point by far more than the tolerance (asserted
+    below for each)."""
+    import scipy.linalg
+
+    S, lb, ub, Wf, _ = c1
+    rp, ci, va = O.csr_of(S)
+    N = O.nullspace(S)
+    W = Wf.shape[1]
+    seed = 8
+    kw = dict(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S.shape[0], P=1, seed=seed, lanes=32)
+    _, st, tr = O.sample(S=step + 1, trace_chains=1, trace_thin=1, **kw)
+    assert st[0, 0] == 0  # no misses: attempt index == step index
+    assert st[0, 1] == st[0, 2] == step // 10 + 1  # every check reprojected
+    c0 = Wf.mean(axis=1)
+    cT = (W * c0 + tr[0, :step].sum(axis=0)) / (W + step)
+    x = c0 if step == 0 else tr[0, step - 1]
+    up = O.stream_u01(seed, step, 0, 0, 0, 0)
+    w = Wf[:, min(int(W * up), W - 1)]
+    u = (w - cT) / np.linalg.norm(w - cT)
+    tmin, tmax = _chord(x, u, lb, ub)
+    r = O.stream_u01(seed, step, 0, 0, 0, 1)
+    y = x + (r * (tmax - tmin) + tmin) * u
+    assert np.abs(S @ y).max() > 1e-9  # the check fires on this point
+    Ns = scipy.linalg.null_space(S)
+    proj = Ns @ (Ns.T @ y)
+    assert np.abs(S @ proj).max() < 1e-12  # the projected point is on S x = 0 before the clip
+    expect = np.clip(proj, lb, ub)
+    tol = 1e-12 * (1 + np.abs(expect).max())
+    np.testing.assert_allclose(tr[0, step], expect, rtol=0, atol=tol)
+    # each plausible mistake lands 100 times that tolerance or more away
+    assert np.abs(np.clip(y, lb, ub) - expect).max() > 100 * tol  # projection skipped
+    for j in (0, Ns.shape[1] // 2, Ns.shape[1] - 1):  # one null-space direction dropped
+        Nd = np.delete(N, j, axis=1)
+        assert np.abs(np.clip(Nd @ (Nd.T @ y), lb, ub) - expect).max() > 100 * tol
+    Nt = N.ravel(order="F").reshape(N.shape, order="C")  # column-major buffer read row-major
+    assert np.abs(np.clip(Nt @ (Nt.T @ y), lb, ub) - expect).max() > 100 * tol
+
+
+def test_reprojection_count_and_value_at_check_period_3(c1):
+    """Checks every 3 steps: steps 0, 3, 6 reproject, the steps between do not
+    (their points are x + t u clipped, off S x = 0 by the step alone)."""
+    S, lb, ub, Wf, _ = c1
+    rp, ci, va = O.csr_of(S)
+    N = O.nullspace(S)
+    kw = dict(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S.shape[0], P=1, seed=2, lanes=16,
+              check_every=3)
+    _, st, tr = O.sample(S=8, trace_chains=1, trace_thin=1, **kw)
+    assert st[0, 2] == 3 and st[0, 1] == 3
+    res = np.abs(S @ tr[0].T).max(axis=0)
+    # a clipped projected point keeps a residual only from its clipped coordinates
+    for k in range(8):
+        on_bound = np.isclose(tr[0, k], lb) | np.isclose(tr[0, k], ub)
+        if k % 3 == 0 and not on_bound.any():
+            assert res[k] < 1e-12
