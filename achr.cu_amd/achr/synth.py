@@ -211,6 +211,32 @@ def chord_warmup(N: np.ndarray, x0: np.ndarray, lb: np.ndarray, ub: np.ndarray, 
     return move_to_center(out)
 
 
+def batch_model(n: int, seed: int, nullspace_fn):
+    """One model of the 64-model batch workload: m = 0.55 n metabolites, the
+    sparsity law above, 30% irreversible, 2% fixed, bounds x0 -+ (5..50).
+    Returns it with an orthonormal basis of its free null space
+    {v : S v = 0, v_i = 0 for fixed i}, both from nullspace_fn(rows, cols,
+    vals, m, n)."""
+    mm = int(round(0.55 * n))
+    sm = make_model(mm, n, seed, irrev_frac=0.30, n_fixed=max(1, int(round(0.02 * n))), width=(5.0, 50.0),
+                    x0_scale=10.0, nullspace_fn=nullspace_fn)
+    fixed = np.nonzero(sm.lb == sm.ub)[0]
+    # free directions: null([S; e_i for the fixed reactions i])
+    r = np.concatenate([sm.rows, mm + np.arange(len(fixed))]).astype(np.int32)
+    c = np.concatenate([sm.cols, fixed]).astype(np.int32)
+    v = np.concatenate([sm.vals, np.ones(len(fixed))])
+    return sm, nullspace_fn(r, c, v, mm + len(fixed), n)
+
+
+def csr(sm: SynthModel):
+    """CSR (rowptr, colidx, vals) of S, columns ascending within a row."""
+    order = np.lexsort((sm.cols, sm.rows))
+    rows, cols, vals = sm.rows[order], sm.cols[order], sm.vals[order]
+    rp = np.zeros(sm.m + 1, dtype=np.int64)
+    np.add.at(rp, rows + 1, 1)
+    return np.cumsum(rp).astype(np.int32), cols.astype(np.int32), vals
+
+
 def round_like_csv(W: np.ndarray) -> np.ndarray:
     """Values as a "%f" warmup CSV stores them (6 decimals)."""
     return np.round(W, 6)

@@ -242,19 +242,10 @@ def gpu_nullspace(rows, cols, vals, m, n):
 
 
 def batch_model(n, seed):
-    """One model of the batch workload: m = 0.55 n metabolites, the sparsity
-    law of the 72 x 95 model, 30% irreversible, 2% fixed, bounds x0 +- (5..50).
-    Returns it with the basis of its free null space (GPU null spaces)."""
+    """One model of the batch workload with the basis of its free null space
+    (GPU null spaces)."""
     from achr import synth
-    mm = int(round(0.55 * n))
-    sm = synth.make_model(mm, n, seed, irrev_frac=0.30, n_fixed=max(1, int(round(0.02 * n))), width=(5.0, 50.0),
-                          x0_scale=10.0, nullspace_fn=gpu_nullspace)
-    fixed = np.nonzero(sm.lb == sm.ub)[0]
-    # free directions: null([S; e_i for the fixed reactions i])
-    r = np.concatenate([sm.rows, mm + np.arange(len(fixed))]).astype(np.int32)
-    c = np.concatenate([sm.cols, fixed]).astype(np.int32)
-    v = np.concatenate([sm.vals, np.ones(len(fixed))])
-    return sm, gpu_nullspace(r, c, v, mm + len(fixed), n)
+    return synth.batch_model(n, seed, gpu_nullspace)
 
 
 def batch_warmup(sm, Nf, seed):
@@ -265,11 +256,8 @@ def batch_warmup(sm, Nf, seed):
 
 
 def csr_of_model(sm):
-    order = np.lexsort((sm.cols, sm.rows))
-    rows, cols, vals = sm.rows[order], sm.cols[order], sm.vals[order]
-    rp = np.zeros(sm.m + 1, dtype=np.int64)
-    np.add.at(rp, rows + 1, 1)
-    return np.cumsum(rp).astype(np.int32), cols.astype(np.int32), vals
+    from achr import synth
+    return synth.csr(sm)
 
 
 def run_batch(args):

@@ -80,3 +80,29 @@ def test_gpu_training_fits_a_target_covariance():
     V = net.sample(Z, prec=2).astype(np.float64)
     emp = V.T @ V / len(V)
     assert np.linalg.norm(emp - C) / np.linalg.norm(C) < 0.15
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_oracle_at_width_512():
+    """The network of the learned workload (hidden width 512, a rank-512
+    principal subspace in and out): GPU forward, loss and gradient against
+    the FP64 oracle, whose gradient the finite-difference test pins."""
+    import achr
+    rng = np.random.default_rng(6)
+    L, H, O, B = 512, 512, 512, 256
+    print(f"dirnet L={L} H={H} O={O} B={B}", flush=True)
+    net = achr.DirNet(L, H, O, seed=9)
+    p = net.params().astype(np.float64)
+    A = rng.standard_normal((O, O))
+    C = (A @ A.T / O).astype(np.float32)
+    net.set_target(C)
+    Z = rng.standard_normal((B, L)).astype(np.float32)
+    V = net.sample(Z, prec=0)
+    Vo, _ = DO.forward(p, Z.astype(np.float64), L, H, O)
+    assert np.linalg.norm(V - Vo) / np.linalg.norm(Vo) < 1e-5
+    np.testing.assert_array_equal(net.sample(-Z, prec=0), -V)
+    loss, g = net.loss_grad(Z)
+    lo, go = DO.loss_grad(p, Z.astype(np.float64), C.astype(np.float64), L, H, O)
+    print(f"loss {loss} vs {lo}; grad rel err {np.linalg.norm(g - go) / np.linalg.norm(go):.3g}", flush=True)
+    assert abs(loss - lo) / lo < 1e-4
+    assert np.linalg.norm(g - go) / np.linalg.norm(go) < 1e-4
