@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -27,28 +28,7 @@
 using achr_host::DevBuf;
 using achr_host::fail;
 
-namespace achr_host {
-
-thread_local std::string g_err;
-
-int fail(int code, const char* fmt, ...) {
-  char buf[1024];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
-}  // namespace achr_host
-
 extern "C" {
-
-const char* achr_last_error(void) { return achr_host::g_err.c_str(); }
-const char* achr_version(void) { return "achr-mi355x 0.1.0 (gfx950)"; }
-
-void achr_free(void* p) { std::free(p); }
 
 void achr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   const achr::u32x4 r = achr::philox4x32_10({ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
@@ -75,451 +55,8 @@ void achr_options_default(achr_options* o) {
   o->model_id = 0;
 }
 
-// ------------------------------------------------------------------ model
 
-static int build_model(int n, int m, int nnz, const int* rows, const int* cols, const double* vals,
-                       const double* lb, const double* ub, achr_model** out) {
-  if (!out) return fail(ACHR_E_ARG, "out is NULL");
-  *out = nullptr;
-  if (n <= 0 || m < 0 || nnz < 0) return fail(ACHR_E_ARG, "bad sizes n=%d m=%d nnz=%d", n, m, nnz);
-  if (nnz > 0 && (!rows || !cols || !vals)) return fail(ACHR_E_ARG, "NULL triplet arrays");
-  if (!lb || !ub) return fail(ACHR_E_ARG, "NULL bounds");
-  std::vector<std::map<int, double>> rowmap(static_cast<size_t>(m));
-  for (int k = 0; k < nnz; ++k) {
-    if (rows[k] < 0 || rows[k] >= m || cols[k] < 0 || cols[k] >= n)
-      return fail(ACHR_E_ARG, "entry %d out of range (row %d col %d)", k, rows[k], cols[k]);
-    rowmap[rows[k]][cols[k]] += vals[k];
-  }
-  auto* M = new (std::nothrow) achr_model();
-  if (!M) return fail(ACHR_E_NOMEM, "model allocation");
-  M->n = n;
-  M->m = m;
-  M->rowptr.assign(static_cast<size_t>(m) + 1, 0);
-  for (int i = 0; i < m; ++i) {
-    for (const auto& kv : rowmap[i]) {
-      if (kv.second == 0.0) continue;
-      M->colidx.push_back(kv.first);
-      M->vals.push_back(kv.second);
-    }
-    M->rowptr[i + 1] = static_cast<int>(M->colidx.size());
-  }
-  M->lb.assign(lb, lb + n);
-  M->ub.assign(ub, ub + n);
-  M->n_file = n;
-  *out = M;
-  return ACHR_OK;
-}
-
-int achr_model_create(int n_rxns, int n_mets, int nnz, const int* rows, const int* cols, const double* vals,
-                      const double* lb, const double* ub, achr_model** out) {
-  return build_model(n_rxns, n_mets, nnz, rows, cols, vals, lb, ub, out);
-}
-
-void achr_model_destroy(achr_model* model) { delete model; }
-
-int achr_model_dims(const achr_model* M, int* n, int* m, int* nnz, int* d) {
-  if (!M) return fail(ACHR_E_ARG, "model is NULL");
-  if (n) *n = M->n;
-  if (m) *m = M->m;
-  if (nnz) *nnz = static_cast<int>(M->vals.size());
-  if (d) *d = M->d;
-  return ACHR_OK;
-}
-
-int achr_model_bounds(const achr_model* M, double* lb, double* ub) {
-  if (!M) return fail(ACHR_E_ARG, "model is NULL");
-  if (lb) std::copy(M->lb.begin(), M->lb.end(), lb);
-  if (ub) std::copy(M->ub.begin(), M->ub.end(), ub);
-  return ACHR_OK;
-}
-
-int achr_model_csr(const achr_model* M, int* rowptr, int* colidx, double* vals) {
-  if (!M) return fail(ACHR_E_ARG, "model is NULL");
-  if (rowptr) std::copy(M->rowptr.begin(), M->rowptr.end(), rowptr);
-  if (colidx) std::copy(M->colidx.begin(), M->colidx.end(), colidx);
-  if (vals) std::copy(M->vals.begin(), M->vals.end(), vals);
-  return ACHR_OK;
-}
-
-// -------------------------------------------------------------- MPS reader
-
-static bool next_tokens(const char* line, std::vector<std::string>& tok) {
-  tok.clear();
-  const char* p = line;
-  while (*p) {
-    while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n') ++p;
-    if (!*p) break;
-    const char* q = p;
-    while (*q && *q != ' ' && *q != '\t' && *q != '\r' && *q != '\n') ++q;
-    tok.emplace_back(p, q);
-    p = q;
-  }
-  return !tok.empty();
-}
-
-static bool parse_num(const std::string& s, double* v) {
-  errno = 0;
-  char* end = nullptr;
-  *v = std::strtod(s.c_str(), &end);
-  return end && *end == '\0' && errno != ERANGE;
-}
-
-// What an MPS file holds, before either view of its rows is taken.
-struct MpsFile {
-  std::vector<std::string> row_names, col_names;
-  std::vector<char> row_type;  // 'E', 'L', 'G', or 'N' for a free row after the objective
-  std::vector<double> rhs, range;  // range: NaN where the RANGES section gives none
-  std::vector<int> rr, cc;
-  std::vector<double> vv, lb, ub;
-};
-
-static int parse_mps(const char* path, MpsFile& F) {
-  if (!path) return fail(ACHR_E_ARG, "NULL argument");
-  FILE* f = std::fopen(path, "r");
-  if (!f) return fail(ACHR_E_IO, "cannot open MPS file %s", path);
-  enum Sec { NONE, ROWS, COLUMNS, RHS, RANGES, BOUNDS, DONE } sec = NONE;
-  std::map<std::string, int> row_index;  // constraint rows only
-  std::string obj_row;
-  std::map<std::string, int> col_index;
-  std::vector<std::string> tok;
-  std::vector<char> buf(1 << 16);
-  int lineno = 0;
-  auto col_of = [&](const std::string& name) {
-    auto it = col_index.find(name);
-    if (it != col_index.end()) return it->second;
-    const int j = static_cast<int>(col_index.size());
-    col_index.emplace(name, j);
-    F.col_names.push_back(name);
-    F.lb.push_back(0.0);
-    F.ub.push_back(DBL_MAX);
-    return j;
-  };
-  int rc = ACHR_OK;
-  while (std::fgets(buf.data(), static_cast<int>(buf.size()), f)) {
-    ++lineno;
-    if (buf[0] == '*') continue;
-    const bool header = buf[0] != ' ' && buf[0] != '\t' && buf[0] != '\n' && buf[0] != '\r';
-    if (!next_tokens(buf.data(), tok)) continue;
-    if (header) {
-      const std::string& h = tok[0];
-      if (h == "NAME") sec = NONE;
-      else if (h == "ROWS") sec = ROWS;
-      else if (h == "COLUMNS") sec = COLUMNS;
-      else if (h == "RHS") sec = RHS;
-      else if (h == "RANGES") sec = RANGES;
-      else if (h == "BOUNDS") sec = BOUNDS;
-      else if (h == "ENDATA") { sec = DONE; break; }
-      else if (h == "OBJSENSE" || h == "OBJSENS") sec = NONE;
-      else { rc = fail(ACHR_E_FORMAT, "%s:%d: unknown section %s", path, lineno, h.c_str()); break; }
-      continue;
-    }
-    if (sec == ROWS) {
-      if (tok.size() < 2) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad ROWS line", path, lineno); break; }
-      const std::string& type = tok[0];
-      if (type == "N" && obj_row.empty()) { obj_row = tok[1]; continue; }
-      if (type != "N" && type != "E" && type != "L" && type != "G") {
-        rc = fail(ACHR_E_FORMAT, "%s:%d: bad row type %s", path, lineno, type.c_str());
-        break;
-      }
-      const int i = static_cast<int>(row_index.size());
-      row_index.emplace(tok[1], i);
-      F.row_names.push_back(tok[1]);
-      F.row_type.push_back(type[0]);
-      F.rhs.push_back(0.0);
-      F.range.push_back(NAN);
-    } else if (sec == COLUMNS) {
-      if (tok.size() >= 3 && (tok[1] == "'MARKER'" || tok[2] == "'MARKER'")) continue;
-      if (tok.size() != 3 && tok.size() != 5) {
-        rc = fail(ACHR_E_FORMAT, "%s:%d: bad COLUMNS line", path, lineno);
-        break;
-      }
-      const int j = col_of(tok[0]);
-      for (size_t k = 1; k + 1 < tok.size(); k += 2) {
-        double v;
-        if (!parse_num(tok[k + 1], &v)) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad number", path, lineno); break; }
-        if (tok[k] == obj_row) continue;
-        auto it = row_index.find(tok[k]);
-        if (it == row_index.end()) {
-          rc = fail(ACHR_E_FORMAT, "%s:%d: unknown row %s", path, lineno, tok[k].c_str());
-          break;
-        }
-        F.rr.push_back(it->second);
-        F.cc.push_back(j);
-        F.vv.push_back(v);
-      }
-      if (rc != ACHR_OK) break;
-    } else if (sec == RHS || sec == RANGES) {
-      // [set name] row value [row value]: the set name is absent when the
-      // line has an even number of fields.
-      const size_t first = tok.size() % 2 == 0 ? 0 : 1;
-      if (tok.size() < 2 || tok.size() > 5) {
-        rc = fail(ACHR_E_FORMAT, "%s:%d: bad %s line", path, lineno, sec == RHS ? "RHS" : "RANGES");
-        break;
-      }
-      for (size_t k = first; k + 1 < tok.size(); k += 2) {
-        double v;
-        if (!parse_num(tok[k + 1], &v)) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad number", path, lineno); break; }
-        if (tok[k] == obj_row) continue;  // minus the objective's constant term
-        auto it = row_index.find(tok[k]);
-        if (it == row_index.end()) {
-          rc = fail(ACHR_E_FORMAT, "%s:%d: unknown row %s", path, lineno, tok[k].c_str());
-          break;
-        }
-        (sec == RHS ? F.rhs : F.range)[it->second] = v;
-      }
-      if (rc != ACHR_OK) break;
-    } else if (sec == BOUNDS) {
-      if (tok.size() < 3) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad BOUNDS line", path, lineno); break; }
-      const std::string& type = tok[0];
-      const bool novalue = type == "FR" || type == "MI" || type == "PL" || type == "BV";
-      // Bound name may be omitted when the line has one field fewer.
-      std::string cname;
-      double v = 0.0;
-      if (novalue) {
-        cname = tok.size() >= 3 ? tok[2] : tok[1];
-      } else {
-        if (tok.size() < 4) { rc = fail(ACHR_E_FORMAT, "%s:%d: missing bound value", path, lineno); break; }
-        cname = tok[2];
-        if (!parse_num(tok[3], &v)) { rc = fail(ACHR_E_FORMAT, "%s:%d: bad number", path, lineno); break; }
-      }
-      auto it = col_index.find(cname);
-      if (it == col_index.end()) {
-        rc = fail(ACHR_E_FORMAT, "%s:%d: unknown column %s", path, lineno, cname.c_str());
-        break;
-      }
-      const int j = it->second;
-      double& lo = F.lb[j];
-      double& hi = F.ub[j];
-      if (type == "UP") {
-        hi = v;
-        if (v < 0.0 && lo == 0.0) lo = -DBL_MAX;  // MPS convention for a negative UP bound
-      } else if (type == "LO") {
-        lo = v;
-      } else if (type == "FX") {
-        lo = v;
-        hi = v;
-      } else if (type == "FR") {
-        lo = -DBL_MAX;
-        hi = DBL_MAX;
-      } else if (type == "MI") {
-        lo = -DBL_MAX;
-      } else if (type == "PL") {
-        hi = DBL_MAX;
-      } else if (type == "BV") {
-        lo = 0.0;
-        hi = 1.0;
-      } else if (type == "LI") {
-        lo = v;
-      } else if (type == "UI") {
-        hi = v;
-      } else {
-        rc = fail(ACHR_E_FORMAT, "%s:%d: bad bound type %s", path, lineno, type.c_str());
-        break;
-      }
-    }
-  }
-  std::fclose(f);
-  if (rc != ACHR_OK) return rc;
-  if (sec != DONE) return fail(ACHR_E_FORMAT, "%s: missing ENDATA", path);
-  if (F.col_names.empty()) return fail(ACHR_E_FORMAT, "%s: no columns", path);
-  return ACHR_OK;
-}
-
-// The reference's view (ACHRCuda.cu:563-636): every row after the objective,
-// whatever its type, is a row of S, and the sampled set is S x = 0; RHS and
-// RANGES are read and ignored.
-int achr_model_read_mps(const char* path, achr_model** out) {
-  if (!path || !out) return fail(ACHR_E_ARG, "NULL argument");
-  *out = nullptr;
-  MpsFile F;
-  int rc = parse_mps(path, F);
-  if (rc != ACHR_OK) return rc;
-  const int n = static_cast<int>(F.col_names.size());
-  const int m = static_cast<int>(F.row_names.size());
-  rc = build_model(n, m, static_cast<int>(F.vv.size()), F.rr.data(), F.cc.data(), F.vv.data(), F.lb.data(),
-                   F.ub.data(), out);
-  if (rc != ACHR_OK) return rc;
-  (*out)->names = std::move(F.col_names);
-  return ACHR_OK;
-}
-
-// The rows as written, the view GLPK gives the reference's warmup generator
-// (createWarmupPtsGLPK.c:219): an E row is S_i x = b_i, an L row S_i x <= b_i,
-// a G row S_i x >= b_i, RANGES give the other side (MPS convention: E with
-// R > 0 is [b, b + R], with R < 0 [b + R, b]; L is [b - |R|, b]; G is
-// [b, b + |R|]), free rows after the objective constrain nothing. The result
-// is again a set {x' : S' x' = 0, lb' <= x' <= ub'} whose first n_file
-// coordinates range over exactly the polytope of the file:
-//   - a row with lo < hi gets a slack column s_i with S_i x - s_i = 0 and
-//     bounds [lo, hi], an infinite side replaced by the row's activity range
-//     over the column bounds (which the row can never leave);
-//   - an equality row with b_i != 0 gets the coefficient -b_i on one column
-//     fixed at 1, appended last.
-int achr_model_read_mps_rows(const char* path, achr_model** out) {
-  if (!path || !out) return fail(ACHR_E_ARG, "NULL argument");
-  *out = nullptr;
-  MpsFile F;
-  int rc = parse_mps(path, F);
-  if (rc != ACHR_OK) return rc;
-  const int n_file = static_cast<int>(F.col_names.size());
-  const int m_file = static_cast<int>(F.row_names.size());
-  constexpr double kInf = 1e30;  // bounds at or beyond this are infinite
-  std::vector<double> act_lo(m_file, 0.0), act_hi(m_file, 0.0);
-  for (size_t k = 0; k < F.vv.size(); ++k) {
-    const double a = F.vv[k], l = F.lb[F.cc[k]], u = F.ub[F.cc[k]];
-    if (a == 0.0) continue;
-    const double lo_term = a > 0 ? (l <= -kInf ? -INFINITY : a * l) : (u >= kInf ? -INFINITY : a * u);
-    const double hi_term = a > 0 ? (u >= kInf ? INFINITY : a * u) : (l <= -kInf ? INFINITY : a * l);
-    act_lo[F.rr[k]] += lo_term;
-    act_hi[F.rr[k]] += hi_term;
-  }
-  std::vector<int> row_new(m_file, -1), aux_row;
-  std::vector<double> aux_lb, aux_ub, rhs_coef(m_file, 0.0);
-  std::vector<std::string> aux_name;
-  int m = 0;
-  bool need_one = false;
-  for (int i = 0; i < m_file; ++i) {
-    const char t = F.row_type[i];
-    if (t == 'N') continue;  // a free row
-    const double b = F.rhs[i], R = F.range[i];
-    const bool ranged = !std::isnan(R);
-    double lo = -INFINITY, hi = INFINITY;
-    if (t == 'E') {
-      lo = hi = b;
-      if (ranged && R > 0) hi = b + R;
-      if (ranged && R < 0) lo = b + R;
-    } else if (t == 'L') {
-      hi = b;
-      if (ranged) lo = b - std::fabs(R);
-    } else {
-      lo = b;
-      if (ranged) hi = b + std::fabs(R);
-    }
-    row_new[i] = m++;
-    if (lo == hi) {
-      if (lo < act_lo[i] || lo > act_hi[i])
-        return fail(ACHR_E_FORMAT, "%s: row %s cannot be met within the column bounds ([%g, %g] excludes %g)", path,
-                    F.row_names[i].c_str(), act_lo[i], act_hi[i], lo);
-      if (b != 0.0) {
-        rhs_coef[i] = -b;
-        need_one = true;
-      }
-      continue;
-    }
-    lo = std::max(lo, act_lo[i]);
-    hi = std::min(hi, act_hi[i]);
-    if (!std::isfinite(lo) || !std::isfinite(hi))
-      return fail(ACHR_E_FORMAT, "%s: row %s is unbounded %s within the column bounds; ACHR needs a bounded set",
-                  path, F.row_names[i].c_str(), std::isfinite(lo) ? "above" : "below");
-    if (lo > hi)
-      return fail(ACHR_E_FORMAT, "%s: row %s cannot be met within the column bounds ([%g, %g])", path,
-                  F.row_names[i].c_str(), lo, hi);
-    aux_row.push_back(row_new[i]);
-    aux_lb.push_back(lo);
-    aux_ub.push_back(hi);
-    aux_name.push_back("slack:" + F.row_names[i]);
-  }
-  const int n_slack = static_cast<int>(aux_row.size());
-  const int n = n_file + n_slack + (need_one ? 1 : 0);
-  std::vector<int> rr, cc;
-  std::vector<double> vv;
-  for (size_t k = 0; k < F.vv.size(); ++k) {
-    if (row_new[F.rr[k]] < 0) continue;
-    rr.push_back(row_new[F.rr[k]]);
-    cc.push_back(F.cc[k]);
-    vv.push_back(F.vv[k]);
-  }
-  for (int k = 0; k < n_slack; ++k) {
-    rr.push_back(aux_row[k]);
-    cc.push_back(n_file + k);
-    vv.push_back(-1.0);
-  }
-  if (need_one) {
-    for (int i = 0; i < m_file; ++i)
-      if (rhs_coef[i] != 0.0) {
-        rr.push_back(row_new[i]);
-        cc.push_back(n - 1);
-        vv.push_back(rhs_coef[i]);
-      }
-    aux_row.push_back(-1);
-    aux_lb.push_back(1.0);
-    aux_ub.push_back(1.0);
-    aux_name.push_back("rhs:1");
-  }
-  std::vector<double> lb(F.lb), ub(F.ub);
-  lb.insert(lb.end(), aux_lb.begin(), aux_lb.end());
-  ub.insert(ub.end(), aux_ub.begin(), aux_ub.end());
-  rc = build_model(n, m, static_cast<int>(vv.size()), rr.data(), cc.data(), vv.data(), lb.data(), ub.data(), out);
-  if (rc != ACHR_OK) return rc;
-  achr_model* M = *out;
-  M->names = std::move(F.col_names);
-  M->names.insert(M->names.end(), aux_name.begin(), aux_name.end());
-  M->n_file = n_file;
-  M->aux_row = std::move(aux_row);
-  return ACHR_OK;
-}
-
-int achr_model_file_columns(const achr_model* M, int* n_file) {
-  if (!M || !n_file) return fail(ACHR_E_ARG, "NULL argument");
-  *n_file = M->n_file;
-  return ACHR_OK;
-}
-
-// Slack k of row r takes the row's activity over the file's columns (CSR
-// order, the slack's own entry skipped); the right-hand-side column is 1.
-int achr_model_complete_points(const achr_model* M, const double* x_file, int P, double* x) {
-  if (!M || (P > 0 && (!x_file || !x))) return fail(ACHR_E_ARG, "NULL argument");
-  if (P < 0) return fail(ACHR_E_ARG, "P=%d", P);
-  const int nf = M->n_file, n = M->n;
-  for (int p = 0; p < P; ++p) {
-    const double* xi = x_file + static_cast<size_t>(p) * nf;
-    double* xo = x + static_cast<size_t>(p) * n;
-    std::copy(xi, xi + nf, xo);
-    for (int k = 0; k < n - nf; ++k) {
-      const int r = M->aux_row[k];
-      double s = 1.0;
-      if (r >= 0) {
-        s = 0.0;
-        for (int q = M->rowptr[r]; q < M->rowptr[r + 1]; ++q)
-          if (M->colidx[q] < nf) s += M->vals[q] * xi[M->colidx[q]];
-      }
-      xo[nf + k] = s;
-    }
-  }
-  return ACHR_OK;
-}
-
-int achr_write_fva_csv(const char* path, const achr_model* M, const double* fva) {
-  if (!path || !M || !fva) return fail(ACHR_E_ARG, "NULL argument");
-  FILE* f = std::fopen(path, "w");
-  if (!f) return fail(ACHR_E_IO, "cannot open %s for writing", path);
-  std::fprintf(f, "reaction,min,max\n");
-  for (int i = 0; i < M->n_file; ++i) {
-    const std::string name = i < static_cast<int>(M->names.size()) ? M->names[i] : "?";
-    std::fprintf(f, "%s,%f,%f\n", name.c_str(), fva[i], fva[static_cast<size_t>(M->n) + i]);
-  }
-  if (std::fclose(f) != 0) return fail(ACHR_E_IO, "write error on %s", path);
-  return ACHR_OK;
-}
-
-// -------------------------------------------------------------- null space
-
-int achr_model_set_nullspace(achr_model* M, const double* N, int d) {
-  if (!M) return fail(ACHR_E_ARG, "model is NULL");
-  if (d < 0 || d > M->n || (d > 0 && !N)) return fail(ACHR_E_ARG, "bad null-space dimension %d", d);
-  M->N.assign(N, N + static_cast<size_t>(M->n) * d);
-  M->d = d;
-  return ACHR_OK;
-}
-
-int achr_model_get_nullspace(const achr_model* M, double* N) {
-  if (!M) return fail(ACHR_E_ARG, "model is NULL");
-  if (M->d < 0) return fail(ACHR_E_STATE, "null space not computed");
-  if (N) std::copy(M->N.begin(), M->N.end(), N);
-  return ACHR_OK;
-}
-
-int achr_model_compute_nullspace(achr_model* M, int* null_dim) {
+static int compute_nullspace(achr_model* M, int* null_dim) {
   if (!M) return fail(ACHR_E_ARG, "model is NULL");
   const int n = M->n;
   // S^T S from the CSR rows on the host: (S^T S)_{ab} = sum_r S_ra S_rb.
@@ -586,6 +123,10 @@ int achr_model_compute_nullspace(achr_model* M, int* null_dim) {
   M->d = d;
   if (null_dim) *null_dim = d;
   return ACHR_OK;
+}
+
+int achr_model_compute_nullspace(achr_model* M, int* null_dim) {
+  return achr_host::no_throw([&] { return compute_nullspace(M, null_dim); });
 }
 
 // ------------------------------------------------------ pseudo-inverse S+
@@ -726,8 +267,8 @@ void achr_sampler_destroy(achr_sampler* S) {
   delete S;
 }
 
-int achr_sampler_create(const achr_model* M, const double* warmup, int W, const achr_options* opts,
-                        achr_sampler** out) {
+static int sampler_create(const achr_model* M, const double* warmup, int W, const achr_options* opts,
+                          achr_sampler** out) {
   if (!M || !warmup || !out) return fail(ACHR_E_ARG, "NULL argument");
   *out = nullptr;
   if (W <= 0) return fail(ACHR_E_ARG, "no warmup points");
@@ -744,7 +285,9 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
   if (o.model_id < 0 || o.model_id >= (1 << 27)) return fail(ACHR_E_ARG, "model_id %d outside 0..2^27-1", o.model_id);
   if (o.proj_prec != ACHR_PROJ_FP64 && o.reproject_mode == 1)
     return fail(ACHR_E_ARG, "reduced-precision projection runs in batched rounds (reproject_mode 2)");
-  auto* S = new (std::nothrow) achr_sampler();
+  // owned here until it is returned: every early exit, an allocation failure
+  // included, releases what the sampler holds
+  std::unique_ptr<achr_sampler, void (*)(achr_sampler*)> S(new (std::nothrow) achr_sampler(), achr_sampler_destroy);
   if (!S) return fail(ACHR_E_NOMEM, "sampler allocation");
   S->model = M;
   S->n = M->n;
@@ -756,16 +299,10 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
   // Groups smaller than a wavefront address the warmup set with 32-bit byte
   // offsets from its base.
   if (S->sched.tpc < 64 && 8.0 * M->n * W >= 2147483648.0) {
-    if (o.lanes != 0) {
-      delete S;
-      return fail(ACHR_E_ARG, "lanes=%d needs n*W*8 < 2 GiB", o.lanes);
-    }
+    if (o.lanes != 0) return fail(ACHR_E_ARG, "lanes=%d needs n*W*8 < 2 GiB", o.lanes);
     S->sched = pick_schedule(M->n, 64);
   }
-  if (S->sched.e < 0) {
-    delete S;
-    return fail(ACHR_E_ARG, "n=%d exceeds the largest register schedule (1024 x 16)", M->n);
-  }
+  if (S->sched.e < 0) return fail(ACHR_E_ARG, "n=%d exceeds the largest register schedule (1024 x 16)", M->n);
   const double nd = static_cast<double>(M->n) * static_cast<double>(M->d);
   S->mode = o.reproject_mode != 0 ? o.reproject_mode : (nd <= 65536.0 && o.proj_prec == ACHR_PROJ_FP64 ? 1 : 2);
   // Warmup centre on the host: (1/W) * sum_k w_k, k ascending (createCenterPt).
@@ -798,91 +335,28 @@ int achr_sampler_create(const achr_model* M, const double* warmup, int W, const 
   }
   if (e == hipSuccess) e = hipEventCreate(&S->ev_begin);
   if (e == hipSuccess) e = hipEventCreate(&S->ev_end);
-  if (e != hipSuccess) {
-    achr_sampler_destroy(S);
-    return fail(ACHR_E_DEVICE, "sampler upload: %s", hipGetErrorString(e));
-  }
+  if (e != hipSuccess) return fail(ACHR_E_DEVICE, "sampler upload: %s", hipGetErrorString(e));
   if (o.proj_prec == ACHR_PROJ_RES_BF16X3 || o.proj_prec == ACHR_PROJ_RES_F32) {
     const int rc = build_pinv_f32(M, S->Sp32);
-    if (rc != ACHR_OK) {
-      achr_sampler_destroy(S);
-      return rc;
-    }
+    if (rc != ACHR_OK) return rc;
   }
-  *out = S;
+  *out = S.release();
   return ACHR_OK;
 }
 
-static int read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_out, int* n_out) {
-  if (!path || !warmup || !W_out) return fail(ACHR_E_ARG, "NULL argument");
-  *warmup = nullptr;
-  FILE* f = std::fopen(path, "r");
-  if (!f) return fail(ACHR_E_IO, "cannot open warmup CSV %s", path);
-  std::vector<std::vector<double>> rows;
-  std::string line;
-  std::vector<char> buf(1 << 20);
-  int W = -1;
-  int rc = ACHR_OK;
-  while (true) {
-    line.clear();
-    bool got = false;
-    while (std::fgets(buf.data(), static_cast<int>(buf.size()), f)) {
-      got = true;
-      line += buf.data();
-      if (!line.empty() && line.back() == '\n') break;
-    }
-    if (!got) break;
-    while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
-    if (line.empty()) continue;
-    std::vector<double> vals;
-    const char* p = line.c_str();
-    while (true) {
-      char* end = nullptr;
-      const double v = std::strtod(p, &end);
-      if (end == p) { rc = fail(ACHR_E_FORMAT, "%s: line %zu: bad number", path, rows.size() + 1); break; }
-      vals.push_back(v);
-      while (*end == ' ' || *end == '\t') ++end;
-      if (*end == ',') { p = end + 1; continue; }
-      if (*end == '\0') break;
-      rc = fail(ACHR_E_FORMAT, "%s: line %zu: unexpected character", path, rows.size() + 1);
-      break;
-    }
-    if (rc != ACHR_OK) break;
-    if (W < 0) W = static_cast<int>(vals.size());
-    if (static_cast<int>(vals.size()) != W) {
-      rc = fail(ACHR_E_FORMAT, "%s: line %zu has %zu values, expected %d", path, rows.size() + 1, vals.size(), W);
-      break;
-    }
-    rows.push_back(std::move(vals));
-  }
-  std::fclose(f);
-  if (rc != ACHR_OK) return rc;
-  const int n = static_cast<int>(rows.size());
-  if (n == 0 || W <= 0) return fail(ACHR_E_FORMAT, "%s: empty warmup file", path);
-  if (n_expected > 0 && n != n_expected)
-    return fail(ACHR_E_FORMAT, "%s: %d lines but the model has %d reactions", path, n, n_expected);
-  auto* w = static_cast<double*>(std::malloc(sizeof(double) * static_cast<size_t>(n) * W));
-  if (!w) return fail(ACHR_E_NOMEM, "warmup allocation");
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < W; ++k) w[static_cast<size_t>(k) * n + i] = rows[i][k];
-  *warmup = w;
-  *W_out = W;
-  if (n_out) *n_out = n;
-  return ACHR_OK;
+int achr_sampler_create(const achr_model* M, const double* warmup, int W, const achr_options* opts,
+                        achr_sampler** out) {
+  return achr_host::no_throw([&] { return sampler_create(M, warmup, W, opts, out); });
 }
 
-int achr_read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_out) {
-  return read_warmup_csv(path, n_expected, warmup, W_out, nullptr);
-}
-
-int achr_sampler_create_csv(const achr_model* M, const char* path, const achr_options* opts,
-                            achr_sampler** out) {
+static int sampler_create_csv(const achr_model* M, const char* path, const achr_options* opts,
+                              achr_sampler** out) {
   if (!M) return fail(ACHR_E_ARG, "model is NULL");
   double* w = nullptr;
   int W = 0, n = 0;
   // A model with auxiliary columns also takes a file of its file columns
   // only (what the reference's generator writes); the rest are completed.
-  int rc = read_warmup_csv(path, M->n_file < M->n ? 0 : M->n, &w, &W, &n);
+  int rc = achr_host::read_warmup_csv(path, M->n_file < M->n ? 0 : M->n, &w, &W, &n);
   if (rc != ACHR_OK) return rc;
   if (n != M->n && n != M->n_file) {
     std::free(w);
@@ -903,6 +377,11 @@ int achr_sampler_create_csv(const achr_model* M, const char* path, const achr_op
   }
   std::free(w);
   return rc;
+}
+
+int achr_sampler_create_csv(const achr_model* M, const char* path, const achr_options* opts,
+                            achr_sampler** out) {
+  return achr_host::no_throw([&] { return sampler_create_csv(M, path, opts, out); });
 }
 
 int achr_sampler_set_stream(achr_sampler* S, void* stream) {
@@ -1149,7 +628,7 @@ int enqueue_segments(achr_sampler* S, const achr::StepArgs& a, int segments, int
     if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
     if (rc != ACHR_OK) return rc;
     HIP_TRY(hipEventRecord(e0, st));
-    const hipError_t le = achr::launch_step(a, S->sched, st);
+    const hipError_t le = achr::launch_step(a, S->run_sched, st);
     if (le != hipSuccess) return fail(ACHR_E_DEVICE, "step kernel launch: %s", hipGetErrorString(le));
     HIP_TRY(hipEventRecord(e1, st));
     S->ev_step.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
@@ -1169,11 +648,16 @@ int enqueue_segments(achr_sampler* S, const achr::StepArgs& a, int segments, int
     }
   }
   HIP_TRY(hipEventRecord(S->ev_end, st));
+  note_run(S, S->run_sched);
+  return ACHR_OK;
+}
+
+void note_run(achr_sampler* S, const achr::Schedule& sc) {
   S->run_use_dirs = S->use_dirs;
   S->run_check_every = S->opt.check_every;
   S->run_trace_thin = S->opt.trace_thin;
   S->run_trace_chains = S->opt.trace_chains;
-  return ACHR_OK;
+  S->run_sched = sc;
 }
 
 }  // namespace achr_host
@@ -1188,6 +672,7 @@ int achr_sample(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   achr::StepArgs a{};
   rc = achr_host::prepare_run(S, P, Ssteps, seed, file, chain0, st, &a);
   if (rc != ACHR_OK) return rc;
+  S->run_sched = S->sched;
   rc = achr_host::enqueue_segments(S, a, achr_host::segments_for(S, Ssteps), 0, st);
   if (rc != ACHR_OK) return rc;
   S->have_run = true;
@@ -1204,7 +689,7 @@ int achr_sample_continue(achr_sampler* S, int S_more) {
   if (S_more == 0) return ACHR_OK;
   if (S->S > INT_MAX - S_more) return fail(ACHR_E_ARG, "too many steps");
   const int S0 = S->S, S1 = S0 + S_more;
-  int rc = achr_host::run_limits(S, S->sched, S->P, S1);
+  int rc = achr_host::run_limits(S, S->run_sched, S->P, S1);
   if (rc != ACHR_OK) return rc;
   hipStream_t st = S->stream;
   if (S->trace_len > 0) {
@@ -1274,7 +759,7 @@ static int save_state(const achr_sampler* S, const char* path) {
   h.n = S->n;
   h.P = S->P;
   h.S = S->S;
-  h.lanes = S->sched.tpc;
+  h.lanes = S->run_sched.tpc;
   h.use_dirs = S->use_dirs;
   h.warm_count = S->use_dirs ? S->K : S->W;
   h.check_every = S->opt.check_every;
@@ -1304,12 +789,14 @@ static int load_state(achr_sampler* S, const char* path) {
     return fail(ACHR_E_FORMAT, "%s is not a sampler state file", path);
   }
   const int warm_count = S->use_dirs ? S->K : S->W;
-  if (h.n != S->n || h.lanes != S->sched.tpc || h.use_dirs != S->use_dirs || h.warm_count != warm_count ||
+  // the saved run's lanes (a batch may have chosen them), if they can hold n
+  const achr::Schedule sc{h.lanes, achr::elems_for(S->n, h.lanes), h.lanes <= 64 ? 256 / std::max(h.lanes, 1) : 1};
+  if (h.n != S->n || !achr::schedule_supported(sc.tpc, sc.e) || h.use_dirs != S->use_dirs || h.warm_count != warm_count ||
       h.check_every != S->opt.check_every || h.mode != S->mode || h.model_id != S->opt.model_id || h.P <= 0 ||
       h.P > (1 << 28) || h.S < 0 || h.chain0 < 0) {
     std::fclose(f);
-    return fail(ACHR_E_STATE, "%s was saved by a sampler with other settings (n, lanes, directions, warmup or bank "
-                "size, check period, reprojection mode or model_id)", path);
+    return fail(ACHR_E_STATE, "%s was saved by a sampler with other settings (n, lanes that cannot hold n, "
+                "directions, warmup or bank size, check period, reprojection mode or model_id)", path);
   }
   const size_t Pn = static_cast<size_t>(h.P) * h.n;
   std::vector<double> X(Pn), C(h.use_dirs ? 0 : Pn);
@@ -1320,7 +807,7 @@ static int load_state(achr_sampler* S, const char* path) {
                   std::fread(stats.data(), sizeof(int32_t), stats.size(), f) == stats.size();
   std::fclose(f);
   if (!ok) return fail(ACHR_E_FORMAT, "%s is truncated", path);
-  int rc = achr_host::run_limits(S, S->sched, h.P, h.S);
+  int rc = achr_host::run_limits(S, sc, h.P, h.S);
   if (rc != ACHR_OK) return rc;
   hipStream_t stream = S->stream;
   achr::StepArgs a{};
@@ -1339,10 +826,7 @@ static int load_state(achr_sampler* S, const char* path) {
   HIP_TRY(hipEventRecord(S->ev_begin, stream));
   HIP_TRY(hipEventRecord(S->ev_end, stream));
   HIP_TRY(hipStreamSynchronize(stream));
-  S->run_use_dirs = S->use_dirs;
-  S->run_check_every = S->opt.check_every;
-  S->run_trace_thin = S->opt.trace_thin;
-  S->run_trace_chains = S->opt.trace_chains;
+  achr_host::note_run(S, sc);
   S->have_run = true;
   return ACHR_OK;
 }
@@ -1453,69 +937,6 @@ int achr_gemm_f32(int M, int N, int K, const float* A, long sa_m, long sa_k, con
   if (e != hipSuccess) return fail(ACHR_E_ARG, "gemm launch: %s", hipGetErrorString(e));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(C, dC.p, sizeof(float) * nc, hipMemcpyDeviceToHost));
-  return ACHR_OK;
-}
-
-int achr_write_points_csv(const char* path, const double* points, int n, int P) {
-  if (!path || !points || n <= 0 || P <= 0) return fail(ACHR_E_ARG, "bad arguments");
-  FILE* f = std::fopen(path, "wb");
-  if (!f) return fail(ACHR_E_IO, "cannot open %s for writing", path);
-  for (int i = 0; i < n; ++i) {
-    for (int j = 0; j < P - 1; ++j) std::fprintf(f, "%f,", points[static_cast<size_t>(i) * P + j]);
-    std::fprintf(f, "%f\n", points[static_cast<size_t>(i) * P + (P - 1)]);
-  }
-  if (std::fclose(f) != 0) return fail(ACHR_E_IO, "write error on %s", path);
-  return ACHR_OK;
-}
-
-namespace {
-constexpr char kPointsMagic[8] = {'A', 'C', 'H', 'R', 'P', 'T', '0', '1'};
-}
-
-int achr_write_points_bin(const char* path, const double* points, int n, int P) {
-  if (!path || !points || n <= 0 || P <= 0) return fail(ACHR_E_ARG, "bad arguments");
-  FILE* f = std::fopen(path, "wb");
-  if (!f) return fail(ACHR_E_IO, "cannot open %s for writing", path);
-  const int32_t dims[2] = {n, P};
-  const size_t cnt = static_cast<size_t>(n) * P;
-  bool ok = std::fwrite(kPointsMagic, 1, 8, f) == 8 && std::fwrite(dims, sizeof(int32_t), 2, f) == 2 &&
-            std::fwrite(points, sizeof(double), cnt, f) == cnt;
-  ok = (std::fclose(f) == 0) && ok;
-  if (!ok) return fail(ACHR_E_IO, "write error on %s", path);
-  return ACHR_OK;
-}
-
-int achr_read_points_bin(const char* path, double** points, int* n, int* P) {
-  if (!path || !points || !n || !P) return fail(ACHR_E_ARG, "NULL argument");
-  *points = nullptr;
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return fail(ACHR_E_IO, "cannot read %s: %s", path, std::strerror(errno));
-  char magic[8];
-  int32_t dims[2];
-  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kPointsMagic, 8) != 0 ||
-      std::fread(dims, sizeof(int32_t), 2, f) != 2 || dims[0] <= 0 || dims[1] <= 0) {
-    std::fclose(f);
-    return fail(ACHR_E_FORMAT, "%s is not a point file", path);
-  }
-  const size_t cnt = static_cast<size_t>(dims[0]) * dims[1];
-  if (cnt > (static_cast<size_t>(1) << 40)) {
-    std::fclose(f);
-    return fail(ACHR_E_FORMAT, "%s: %d x %d points is not a plausible size", path, dims[0], dims[1]);
-  }
-  double* buf = static_cast<double*>(std::malloc(sizeof(double) * cnt));
-  if (!buf) {
-    std::fclose(f);
-    return fail(ACHR_E_NOMEM, "out of host memory");
-  }
-  const bool ok = std::fread(buf, sizeof(double), cnt, f) == cnt;
-  std::fclose(f);
-  if (!ok) {
-    std::free(buf);
-    return fail(ACHR_E_FORMAT, "%s is truncated", path);
-  }
-  *points = buf;
-  *n = dims[0];
-  *P = dims[1];
   return ACHR_OK;
 }
 

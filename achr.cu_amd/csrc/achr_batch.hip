@@ -349,9 +349,11 @@ int achr_batch_sample(achr_batch* B, const int* P, int Ssteps, uint64_t seed, ui
   }
   B->graphs = static_cast<int>(graphs.size());
   if (rc == ACHR_OK) {
-    for (auto* S : B->s) {
+    for (int k = 0; k < count; ++k) {
+      achr_sampler* S = B->s[k];
       hipError_t e = hipEventRecord(S->ev_end, B->stream);
       if (e != hipSuccess) rc = fail(ACHR_E_DEVICE, "event: %s", hipGetErrorString(e));
+      achr_host::note_run(S, B->sched[k]);
       S->have_run = true;
     }
   }

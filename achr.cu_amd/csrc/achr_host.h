@@ -12,10 +12,9 @@
 
 #include "../../include/achr.h"
 #include "achr_internal.h"
+#include "achr_model.h"
 
 namespace achr_host {
-
-int fail(int code, const char* fmt, ...);
 
 template <typename T>
 struct DevBuf {
@@ -69,25 +68,15 @@ struct DevBuf {
     if (e_ != hipSuccess) return achr_host::fail(ACHR_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-struct achr_model {
-  int n = 0, m = 0;
-  std::vector<int> rowptr, colidx;
-  std::vector<double> vals, lb, ub;
-  std::vector<double> N;  // n x d column major
-  std::vector<std::string> names;  // reaction (column) names of an MPS model
-  int d = -1;             // -1: not computed yet
-  // Columns n_file..n-1 of a model read with its rows honoured are auxiliary:
-  // aux_row[k] is the row whose activity column n_file + k carries (its slack),
-  // or -1 for the column fixed at 1 that carries the nonzero right-hand sides.
-  int n_file = 0;
-  std::vector<int> aux_row;
-};
-
 struct achr_sampler {
   const achr_model* model = nullptr;
   int n = 0, m = 0, W = 0, d = 0;
   achr_options opt{};
   achr::Schedule sched{};
+  // Schedule of the last run: its lanes fix the norm's reduction tree, so a
+  // continuation or checkpoint of that run keeps them (a batch may have run
+  // the sampler's chains with other lanes than sched).
+  achr::Schedule run_sched{};
   int mode = 1;
   hipStream_t stream = nullptr;
   std::vector<double> c0;
@@ -124,19 +113,6 @@ struct achr_sampler {
 
 namespace achr_host {
 
-// Runs f() and turns a host allocation failure into ACHR_E_NOMEM, for entry
-// points whose host buffers are sized by their inputs (files, models).
-template <class F>
-int no_throw(F&& f) {
-  try {
-    return f();
-  } catch (const std::bad_alloc&) {
-    return fail(ACHR_E_NOMEM, "out of host memory");
-  } catch (const std::length_error&) {
-    return fail(ACHR_E_NOMEM, "host buffer too large");
-  }
-}
-
 // Run set-up and projection rounds shared by single-model runs and batches.
 enum { kProjNone = 0, kProjFP64 = 1, kProjLow = 2 };
 // Rejects run sizes a schedule cannot address.
@@ -150,12 +126,16 @@ int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
 // Kernel arguments of a run of Ssteps steps over the sampler's current buffers.
 void fill_args(const achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file, int chain0,
                achr::StepArgs* args);
-// Queues the segments [0, segments) of a run (step launch, then the projection
-// round of checks already done + segment index), recording their events.
+// Queues the segments [0, segments) of a run with the schedule S->run_sched
+// (step launch, then the projection round of checks already done + segment
+// index), recording their events.
 int enqueue_segments(achr_sampler* S, const achr::StepArgs& a, int segments, int checks_done, hipStream_t st);
 // Projection round after segment seg (kProjNone outside mode 2).
 int projection_kind(const achr_sampler* S, int seg);
 // Queues one projection round over the chains flagged by the last segment.
 int enqueue_projection(achr_sampler* S, int kind, hipStream_t st);
+// Records the settings of the run just queued (directions, check period,
+// trace, schedule) that achr_sample_continue and the checkpoint must match.
+void note_run(achr_sampler* S, const achr::Schedule& sc);
 
 }  // namespace achr_host

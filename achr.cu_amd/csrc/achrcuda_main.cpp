@@ -21,6 +21,7 @@
 //                          ranges (the set createWarmupPts solves over) instead
 //                          of sampling S x = 0 over every row as the reference
 //                          does; File<k> still holds the file's columns only
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -102,9 +103,17 @@ int main(int argc, char** argv) {
   std::vector<double> pts(static_cast<size_t>(n) * P);
   for (long k = 0; k < nFiles; ++k) {
     std::printf("File %ld\n", k);
-    if (achr_sample(sampler, static_cast<int>(P), static_cast<int>(S), seed, static_cast<uint32_t>(k), 0) != ACHR_OK)
-      return die("sampling");
-    if (achr_sample_points(sampler, pts.data()) != ACHR_OK) return die("fetching points");
+    if (S == 0) {
+      // The reference clears its points buffer before every file
+      // (ACHRCuda.cu:739) and writes a chain's point only inside the step
+      // loop (:322, :350), so a file of zero steps holds zeros. (The library's
+      // achr_sample returns each chain's start point instead.)
+      std::fill(pts.begin(), pts.end(), 0.0);
+    } else {
+      if (achr_sample(sampler, static_cast<int>(P), static_cast<int>(S), seed, static_cast<uint32_t>(k), 0) != ACHR_OK)
+        return die("sampling");
+      if (achr_sample_points(sampler, pts.data()) != ACHR_OK) return die("fetching points");
+    }
     const std::string name = "File" + std::to_string(k);
     // reaction major: the file's columns are the first nf rows
     if (achr_write_points_csv(name.c_str(), pts.data(), nf, static_cast<int>(P)) != ACHR_OK)

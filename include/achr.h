@@ -189,6 +189,8 @@ int achr_sample(achr_sampler *sampler, int P, int S, uint64_t seed, uint32_t fil
 /* Continues every chain of the last run (or of a loaded state) for S_more
  * further accepted steps, as if the run had been started with S + S_more:
  * points, counters and traces equal those of the longer run bit for bit.
+ * The run keeps its threads per chain, also when a batch chose them
+ * (achr_batch_sample, achr_batch_set_lanes).
  * The directions (bank or ACHR), check period and trace settings must be
  * those of the run. No counterpart in the reference CLI (one fixed S per
  * file); it is the resume half of the MATLAB sampler's time-limited runs
@@ -196,9 +198,10 @@ int achr_sample(achr_sampler *sampler, int P, int S, uint64_t seed, uint32_t fil
 int achr_sample_continue(achr_sampler *sampler, int S_more);
 
 /* Checkpoint of the last run's chains (points, centres, counters, seed,
- * file, chain offset and step count) to a binary file, and its reload into
- * a sampler of the same model, warmup or bank, lanes, check period,
- * reprojection mode and model_id (ACHR_E_STATE otherwise); after a reload
+ * file, chain offset, step count and threads per chain) to a binary file,
+ * and its reload into a sampler of the same model, warmup or bank, check
+ * period, reprojection mode and model_id whose n the saved threads per chain
+ * can hold (ACHR_E_STATE otherwise); after a reload
  * achr_sample_points returns the saved points and achr_sample_continue
  * resumes the chains exactly. Traces are not part of the state. */
 int achr_sampler_save_state(const achr_sampler *sampler, const char *path);

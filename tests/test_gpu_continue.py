@@ -102,3 +102,49 @@ def test_several_continues_and_refusals(setup, tmp_path):
     (tmp_path / "bad.bin").write_bytes(b"not a state file at all, nothing here to read")
     with pytest.raises(achr.AchrError):
         make(setup, 64, 2, False).load_state(str(tmp_path / "bad.bin"))
+
+
+def test_batch_run_continues_and_resumes_with_its_own_lanes(setup, tmp_path):
+    """A batch may run a sampler's chains with other lanes than the sampler's
+    own schedule (pinned or autotuned); the lanes fix the norm's reduction
+    tree, so continuing such a run, or saving it and resuming it in a fresh
+    sampler, must keep them: S1 steps in the batch then S2 more equal one
+    batch run of S1 + S2 steps, bit for bit, and the oracle at those lanes."""
+    P, seed, file, S1, S2 = 24, 8, 1, 30, 27
+    lanes = [64, 256]
+
+    def pair():
+        out = []
+        for k in range(2):
+            smp = achr.Sampler(setup["model"], setup["W"], achr.Options(model_id=k + 1, reproject_mode=2))
+            out.append(smp)
+        assert all(s.schedule()[0] == 32 for s in out)  # their own schedule: 32 lanes
+        return out
+
+    longs = pair()
+    b = achr.Batch(longs)
+    b.set_lanes(lanes)
+    b.run([P, P], S1 + S2, seed=seed, file=file)
+    want = [(s.points(), s.stats()) for s in longs]
+    for k in range(2):
+        o, ost, _ = O.sample(warm=setup["W"], lb=setup["lb"], ub=setup["ub"], rowptr=setup["rp"],
+                             colidx=setup["ci"], vals=setup["va"], N=setup["N"], m=setup["S"].shape[0], P=P,
+                             S=S1 + S2, seed=seed, file=file, model=k + 1, lanes=lanes[k], threads=8)
+        np.testing.assert_array_equal(want[k][1], ost)
+        np.testing.assert_array_equal(want[k][0], o)
+    shorts = pair()
+    b2 = achr.Batch(shorts)
+    b2.set_lanes(lanes)
+    b2.run([P, P], S1, seed=seed, file=file)
+    path = tmp_path / "batch1.state"
+    shorts[1].save_state(str(path))
+    say("continue model 0 in place")
+    shorts[0].continue_run(S2)
+    np.testing.assert_array_equal(shorts[0].stats(), want[0][1])
+    np.testing.assert_array_equal(shorts[0].points(), want[0][0])
+    say("resume model 1 in a fresh sampler")
+    fresh = achr.Sampler(setup["model"], setup["W"], achr.Options(model_id=2, reproject_mode=2))
+    fresh.load_state(str(path))
+    fresh.continue_run(S2)
+    np.testing.assert_array_equal(fresh.stats(), want[1][1])
+    np.testing.assert_array_equal(fresh.points(), want[1][0])

@@ -69,9 +69,12 @@ def test_batched_projection_bitwise(c1_setup, lanes):
 def test_edge_sizes_one_chain_ragged_and_zero_steps(c1_setup, lanes, mode):
     """One chain, chain counts that fill no whole block, and zero steps. The
     reference starts each chain at the warmup centre (ACHRCuda.cu:374-377) and
-    writes its point only inside the step loop (:349), so with stepsPerPoint 0
-    its File<k> holds whatever the buffer held; here a chain with no steps
-    returns its start, the centre. A run of no chains is refused."""
+    writes its point only inside the step loop (:322, :350) into a buffer it
+    clears before every file (:739), so with stepsPerPoint 0 its File<k> holds
+    zeros, which the ACHRCuda command line here reproduces
+    (test_cli_zero_steps_writes_zeros). The library's achr_sample returns a
+    chain's start point, the centre, after zero steps. A run of no chains is
+    refused."""
     for P, S in [(1, 150), (3, 57), (5, 0)]:
         g, gs, _, o, os_, _ = run_both(c1_setup, c1_setup["Wf"], lanes=lanes, mode=mode, P=P, S=S, seed=31)
         np.testing.assert_array_equal(gs, os_)
@@ -189,6 +192,22 @@ def test_cli_drop_in(tmp_path, c1_setup):
     r = subprocess.run([achr.CLI_PATH, "missing.mps", "w.csv", "1", "1", "1"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0 and "MPS" in r.stderr
+
+
+def test_cli_zero_steps_writes_zeros(tmp_path, c1_setup):
+    """ACHRCuda m.mps w.csv 2 7 0: the reference clears its points buffer
+    before each file (ACHRCuda.cu:739) and its step loop never runs, so each
+    File<k> is n lines of P zeros in "%f"."""
+    npath = tmp_path / "N.bin"
+    np.asfortranarray(c1_setup["N"]).ravel(order="F").tofile(npath)
+    env = dict(os.environ, ACHR_NULLSPACE=str(npath))
+    say("cli run, zero steps")
+    r = subprocess.run([achr.CLI_PATH, golden("c1.mps"), golden("c1_warmup.csv"), "2", "7", "0", "--lossless"],
+                       cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for f in range(2):
+        assert (tmp_path / f"File{f}").read_text() == ("0.000000," * 6 + "0.000000\n") * 95
+        np.testing.assert_array_equal(achr.read_points_bin(tmp_path / f"File{f}.bin"), np.zeros((95, 7)))
 
 
 @pytest.mark.parametrize("mets,rxns,lanes", [(4900, 9000, 1024), (3800, 7000, 512)])
