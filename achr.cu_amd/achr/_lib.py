@@ -54,6 +54,8 @@ SIGNATURES = {
     "achr_sampler_destroy": (None, [_P]),
     "achr_sampler_set_stream": (_I, [_P, _P]),
     "achr_sampler_center": (_I, [_P, _PD]),
+    "achr_sampler_project_warmup": (_I, [_P]),
+    "achr_sampler_get_warmup": (_I, [_P, _PD, _PI]),
     "achr_sample": (_I, [_P, _I, _I, C.c_uint64, C.c_uint32, _I]),
     "achr_sample_points": (_I, [_P, _PD]),
     "achr_sample_points_device": (_I, [_P, C.POINTER(_P)]),

@@ -207,6 +207,20 @@ class Sampler:
         check(lib().achr_sampler_center(self._h, _dp(c)), "center")
         return c
 
+    def project_warmup(self) -> None:
+        """Moves the warmup points onto {S x = 0, fixed reactions at their bound}
+        and recomputes the centre (achr_sampler_project_warmup)."""
+        check(lib().achr_sampler_project_warmup(self._h), "achr_sampler_project_warmup")
+
+    def warmup(self) -> np.ndarray:
+        """The sampler's warmup set, n x W."""
+        W = C.c_int()
+        check(lib().achr_sampler_get_warmup(self._h, None, C.byref(W)), "achr_sampler_get_warmup")
+        out = np.empty((self.n, W.value), order="F")
+        check(lib().achr_sampler_get_warmup(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(W)),
+              "achr_sampler_get_warmup")
+        return out
+
     def schedule(self):
         lanes, mode = C.c_int(), C.c_int()
         check(lib().achr_sampler_schedule(self._h, C.byref(lanes), C.byref(mode)), "schedule")

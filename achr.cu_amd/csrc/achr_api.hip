@@ -133,56 +133,60 @@ int achr_model_compute_nullspace(achr_model* M, int* null_dim) {
 
 // S+ = S^T (S S^T)+ with the eigen-decomposition of S S^T (rocSOLVER syevd,
 // eigenvalues below m * lambda_max * 2.2204e-16 dropped, the rule the
-// reference applies to S^T S) and one rocBLAS DGEMM; stored as FP32.
-// x - S+ (S x) is the orthogonal projection of x onto null(S), the same
-// operator as N N^T x.
+// reference applies to S^T S), (S S^T)+ by one rocBLAS DGEMM, then S^T times
+// it straight from the sparse columns of S (launch_pinv_csc); stored as FP32.
+// S itself is never made dense. x - S+ (S x) is the orthogonal projection of
+// x onto null(S), the same operator as N N^T x.
 static int build_pinv_f32(const achr_model* M, DevBuf<float>& out) {
   const int n = M->n, m = M->m;
   if (m == 0) {
     HIP_TRY(out.ensure(1));
     return ACHR_OK;
   }
-  // S S^T on the host from the columns of S.
-  std::vector<std::vector<std::pair<int, double>>> colent(static_cast<size_t>(n));
+  // Columns of S (CSC): rows ascending within a column.
+  std::vector<int> colptr(static_cast<size_t>(n) + 1, 0);
   for (int r = 0; r < m; ++r)
-    for (int p = M->rowptr[r]; p < M->rowptr[r + 1]; ++p) colent[M->colidx[p]].emplace_back(r, M->vals[p]);
+    for (int p = M->rowptr[r]; p < M->rowptr[r + 1]; ++p) ++colptr[M->colidx[p] + 1];
+  for (int i = 0; i < n; ++i) colptr[i + 1] += colptr[i];
+  std::vector<int> rowidx(M->vals.size() > 0 ? M->vals.size() : 1), fillp(colptr.begin(), colptr.end() - 1);
+  std::vector<double> cvals(rowidx.size(), 0.0);
+  for (int r = 0; r < m; ++r)
+    for (int p = M->rowptr[r]; p < M->rowptr[r + 1]; ++p) {
+      const int q = fillp[M->colidx[p]]++;
+      rowidx[q] = r;
+      cvals[q] = M->vals[p];
+    }
+  // S S^T on the host from the columns of S.
   std::vector<double> G(static_cast<size_t>(m) * m, 0.0);
   for (int j = 0; j < n; ++j)
-    for (const auto& a : colent[j])
-      for (const auto& b : colent[j]) G[static_cast<size_t>(a.first) + static_cast<size_t>(b.first) * m] += a.second * b.second;
-  std::vector<double> Sd(static_cast<size_t>(m) * n, 0.0);  // dense S, m x n column major
-  for (int r = 0; r < m; ++r)
-    for (int p = M->rowptr[r]; p < M->rowptr[r + 1]; ++p)
-      Sd[static_cast<size_t>(r) + static_cast<size_t>(M->colidx[p]) * m] = M->vals[p];
+    for (int a = colptr[j]; a < colptr[j + 1]; ++a)
+      for (int b = colptr[j]; b < colptr[j + 1]; ++b)
+        G[static_cast<size_t>(rowidx[a]) + static_cast<size_t>(rowidx[b]) * m] += cvals[a] * cvals[b];
   rocblas_handle h = nullptr;
   if (rocblas_create_handle(&h) != rocblas_status_success) return fail(ACHR_E_SOLVER, "rocblas_create_handle");
-  double *dG = nullptr, *dD = nullptr, *dE = nullptr, *dW = nullptr, *dS = nullptr, *dP = nullptr;
-  rocblas_int* dinfo = nullptr;
+  DevBuf<double> dG, dD, dE, dW, dGinv, dv;
+  DevBuf<int> dinfo, dcp, dri;
   int rc = ACHR_OK;
   std::vector<double> ev(m);
   do {
     const size_t mm = static_cast<size_t>(m) * m, mn = static_cast<size_t>(m) * n;
-    if (hipMalloc(&dG, 8 * mm) != hipSuccess || hipMalloc(&dD, 8 * static_cast<size_t>(m)) != hipSuccess ||
-        hipMalloc(&dE, 8 * static_cast<size_t>(m)) != hipSuccess || hipMalloc(&dinfo, sizeof(rocblas_int)) != hipSuccess ||
-        hipMalloc(&dW, 8 * mm) != hipSuccess || hipMalloc(&dS, 8 * mn) != hipSuccess ||
-        hipMalloc(&dP, 8 * mn) != hipSuccess) {
-      rc = fail(ACHR_E_NOMEM, "device allocation for the pseudo-inverse");
+    if (dG.upload(G.data(), mm, nullptr) != hipSuccess || dD.ensure(m) != hipSuccess || dE.ensure(m) != hipSuccess ||
+        dinfo.ensure(1) != hipSuccess || dW.ensure(mm) != hipSuccess || dGinv.ensure(mm) != hipSuccess ||
+        dcp.upload(colptr.data(), colptr.size(), nullptr) != hipSuccess ||
+        dri.upload(rowidx.data(), rowidx.size(), nullptr) != hipSuccess ||
+        dv.upload(cvals.data(), cvals.size(), nullptr) != hipSuccess) {
+      rc = fail(ACHR_E_NOMEM, "device buffers for the pseudo-inverse");
       break;
     }
-    if (hipMemcpy(dG, G.data(), 8 * mm, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dS, Sd.data(), 8 * mn, hipMemcpyHostToDevice) != hipSuccess) {
-      rc = fail(ACHR_E_DEVICE, "upload S S^T / S");
-      break;
-    }
-    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, m, dG, m, dD, dE, dinfo) !=
+    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, m, dG.p, m, dD.p, dE.p, dinfo.p) !=
         rocblas_status_success) {
       rc = fail(ACHR_E_SOLVER, "rocsolver_dsyevd (S S^T)");
       break;
     }
     rocblas_int info = 0;
-    if (hipMemcpy(&info, dinfo, sizeof info, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(ev.data(), dD, 8 * static_cast<size_t>(m), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(G.data(), dG, 8 * mm, hipMemcpyDeviceToHost) != hipSuccess || info != 0) {
+    if (hipMemcpy(&info, dinfo.p, sizeof info, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ev.data(), dD.p, 8 * static_cast<size_t>(m), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(G.data(), dG.p, 8 * mm, hipMemcpyDeviceToHost) != hipSuccess || info != 0) {
       rc = fail(ACHR_E_SOLVER, "syevd (S S^T) info %d", static_cast<int>(info));
       break;
     }
@@ -195,43 +199,22 @@ static int build_pinv_f32(const achr_model* M, DevBuf<float>& out) {
       const double sc = ev[k] >= tol ? 1.0 / ev[k] : 0.0;
       for (int i = 0; i < m; ++i) Us[static_cast<size_t>(i) + static_cast<size_t>(k) * m] = G[static_cast<size_t>(i) + static_cast<size_t>(k) * m] * sc;
     }
-    if (hipMemcpy(dW, Us.data(), 8 * mm, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(dW.p, Us.data(), 8 * mm, hipMemcpyHostToDevice) != hipSuccess) {
       rc = fail(ACHR_E_DEVICE, "upload U/lambda");
       break;
     }
     const double one = 1.0, zero = 0.0;
-    // dE is reused only as scratch below; Ginv = (U/lambda) U^T into dG's slot needs U itself: dG holds U.
-    double* dGinv = nullptr;
-    if (hipMalloc(&dGinv, 8 * mm) != hipSuccess) {
-      rc = fail(ACHR_E_NOMEM, "device allocation (S S^T)+");
-      break;
-    }
-    rocblas_status st1 = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, m, m, &one, dW, m,
-                                       dG, m, &zero, dGinv, m);
-    // S+ = S^T Ginv : (n x m) = (m x n)^T (m x m)
-    rocblas_status st2 = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, m, m, &one, dS, m,
-                                       dGinv, m, &zero, dP, n);
-    (void)hipFree(dGinv);
-    if (st1 != rocblas_status_success || st2 != rocblas_status_success) {
-      rc = fail(ACHR_E_SOLVER, "rocblas_dgemm for S+");
-      break;
-    }
-    if (hipDeviceSynchronize() != hipSuccess) {
-      rc = fail(ACHR_E_DEVICE, "pseudo-inverse");
+    // (S S^T)+ = (U / lambda) U^T; dG holds U
+    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, m, m, &one, dW.p, m, dG.p, m, &zero,
+                      dGinv.p, m) != rocblas_status_success) {
+      rc = fail(ACHR_E_SOLVER, "rocblas_dgemm for (S S^T)+");
       break;
     }
     hipError_t e = out.ensure(mn);
-    if (e == hipSuccess) e = achr::launch_to_f32(dP, out.p, mn, nullptr);
+    if (e == hipSuccess) e = achr::launch_pinv_csc(n, m, dcp.p, dri.p, dv.p, dGinv.p, out.p, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) rc = fail(ACHR_E_DEVICE, "S+ to FP32: %s", hipGetErrorString(e));
+    if (e != hipSuccess) rc = fail(ACHR_E_DEVICE, "S+ from the columns of S: %s", hipGetErrorString(e));
   } while (false);
-  (void)hipFree(dG);
-  (void)hipFree(dD);
-  (void)hipFree(dE);
-  (void)hipFree(dinfo);
-  (void)hipFree(dW);
-  (void)hipFree(dS);
-  (void)hipFree(dP);
   rocblas_destroy_handle(h);
   return rc;
 }
@@ -519,6 +502,7 @@ void fill_args(const achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t
   a.check_every = S->opt.check_every;
   a.max_misses = S->opt.max_misses;
   a.mode = S->mode;
+  a.stop_flagged_only = S->mode == 2 && S->opt.proj_prec == ACHR_PROJ_FP64;
   a.X = S->X.p;
   a.C = S->C.p;
   a.T = S->T.p;

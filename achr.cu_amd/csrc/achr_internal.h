@@ -26,6 +26,7 @@ struct StepArgs {
   double dTol, uTol, maxMinTol, reprojTol;
   int check_every, max_misses;
   int mode;              // 1: reprojection inside the kernel; 2: stop at checks (batched GEMM)
+  int stop_flagged_only; // mode 2: a chain stops only at a check that flags it; the others run on
   // state, chain major
   double* X;             // P x n: x of chain b at X + b*n
   double* C;             // P x n: running centre
@@ -84,6 +85,15 @@ hipError_t launch_axpby(float* y, const float* x, float alpha, float beta, size_
 hipError_t launch_to_f64(const float* src, double* dst, size_t count, hipStream_t stream);
 // Sum of squares of `count` FP32 values, accumulated in FP64 (synchronous).
 hipError_t sum_squares(const float* x, size_t count, double* out);
+// Y = X - v 1^T over the cols columns of the n x cols matrix X.
+hipError_t launch_sub_cols(const double* X, const double* v, double* Y, int n, int cols, hipStream_t stream);
+// X += v 1^T, then rows with lb == ub set to lb.
+hipError_t launch_add_cols_fix(double* X, const double* v, const double* lb, const double* ub, int n, int cols,
+                               hipStream_t stream);
+// S+ = S^T G (n x m column major, FP32) from the CSC form of S (colptr n+1,
+// rowidx / vals per column) and the symmetric m x m FP64 matrix G.
+hipError_t launch_pinv_csc(int n, int m, const int* colptr, const int* rowidx, const double* vals, const double* G,
+                           float* out, hipStream_t stream);
 // Elementwise FP64 -> FP32 copy of `count` values.
 hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_t stream);
 // r[i] = the bank-mode chord's reciprocal of u[i] (device arrays).

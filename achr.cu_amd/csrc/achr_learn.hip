@@ -290,6 +290,68 @@ static int compute_bank_start(achr_sampler* S) {
   return ACHR_OK;
 }
 
+// Warmup points onto {x : S x = 0, x_F = lb_F}: w <- s0 + N_f N_f^T (w - s0)
+// with s0 the bank start (a point of that set) and N_f the orthonormal basis
+// of its directions, both FP64 products on the matrix cores in the oracle's
+// dot-product order; the fixed coordinates are then set to their bound
+// exactly. The centre is recomputed from the moved points as at creation.
+static int free_basis(const achr_model* M, const double* dN, DevBuf<double>& Nf, int* df_out);
+
+static int project_warmup(achr_sampler* S) {
+  const achr_model* M = S->model;
+  const int n = S->n, W = S->W;
+  if (S->d <= 0) return fail(ACHR_E_STATE, "the model has no null space to project onto");
+  if (S->s0.empty()) {
+    const int rc = compute_bank_start(S);
+    if (rc != ACHR_OK) return rc;
+  }
+  DevBuf<double> Nf, D, T;
+  int df = 0;
+  int rc = free_basis(M, S->Ncm.p, Nf, &df);
+  if (rc != ACHR_OK) return rc;
+  HIP_TRY(D.ensure(static_cast<size_t>(n) * W));
+  HIP_TRY(T.ensure(static_cast<size_t>(df) * W));
+  DevBuf<int> wcount;
+  HIP_TRY(wcount.upload(&W, 1, nullptr));
+  HIP_TRY(achr::launch_sub_cols(S->warm.p, S->s0d.p, D.p, n, W, nullptr));
+  // T(:, k) = N_f^T D(:, k): A(r, i) = N_f[i + r*n]
+  HIP_TRY(achr::launch_dgemm_cols(df, n, Nf.p, n, 1, D.p, n, T.p, df, wcount.p, W, nullptr));
+  // W(:, k) = N_f T(:, k) + s0: A(i, r) = N_f[i + r*n]
+  HIP_TRY(achr::launch_dgemm_cols(n, df, Nf.p, 1, n, T.p, df, S->warm.p, n, wcount.p, W, nullptr));
+  HIP_TRY(achr::launch_add_cols_fix(S->warm.p, S->s0d.p, S->lb.p, S->ub.p, n, W, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  // the centre of the moved points, summed in point order as at creation
+  std::vector<double> sum(n, 0.0), chunk;
+  const int per = std::max(1, static_cast<int>((64LL << 20) / (8LL * n)));
+  for (int k0 = 0; k0 < W; k0 += per) {
+    const int cnt = std::min(per, W - k0);
+    chunk.resize(static_cast<size_t>(cnt) * n);
+    HIP_TRY(hipMemcpy(chunk.data(), S->warm.p + static_cast<size_t>(k0) * n, sizeof(double) * chunk.size(),
+                      hipMemcpyDeviceToHost));
+    for (int k = 0; k < cnt; ++k)
+      for (int i = 0; i < n; ++i) sum[i] += chunk[static_cast<size_t>(k) * n + i];
+  }
+  const double alpha = 1.0 / static_cast<double>(W);
+  for (int i = 0; i < n; ++i) S->c0[i] = alpha * sum[i];
+  HIP_TRY(S->c0d.upload(S->c0.data(), static_cast<size_t>(n), nullptr));
+  S->s0.clear();  // the bank start follows the new centre when a bank is switched on
+  S->have_run = false;
+  return ACHR_OK;
+}
+
+int achr_sampler_project_warmup(achr_sampler* S) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  return achr_host::no_throw([&] { return project_warmup(S); });
+}
+
+int achr_sampler_get_warmup(const achr_sampler* S, double* warmup, int* W) {
+  if (!S || !W) return fail(ACHR_E_ARG, "NULL argument");
+  *W = S->W;
+  if (warmup)
+    HIP_TRY(hipMemcpy(warmup, S->warm.p, sizeof(double) * static_cast<size_t>(S->n) * S->W, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
 int achr_sampler_use_directions(achr_sampler* S, int on) {
   if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
   if (on && S->K == 0) return fail(ACHR_E_STATE, "no direction bank");

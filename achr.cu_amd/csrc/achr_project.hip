@@ -364,6 +364,73 @@ hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_
   return hipGetLastError();
 }
 
+namespace {
+// S+ (n x m column major, FP32) = S^T G with G = (S S^T)+ symmetric (m x m):
+// S+[i + k*n] = sum over the entries (r, v) of column i of S of v * G[r + k*m],
+// accumulated in FP64 in the column's row order, rounded once. Threads run
+// over i, so stores are coalesced; the reads of G gather within one column.
+__global__ __launch_bounds__(256) void pinv_csc_kernel(int n, int m, const int* __restrict__ colptr,
+                                                       const int* __restrict__ rowidx, const double* __restrict__ vals,
+                                                       const double* __restrict__ G, float* __restrict__ out) {
+  const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+  const int k = static_cast<int>(blockIdx.y);
+  if (i >= n) return;
+  const double* Gk = G + static_cast<size_t>(k) * m;
+  double acc = 0.0;
+  for (int p = colptr[i]; p < colptr[i + 1]; ++p) acc = fma(vals[p], Gk[rowidx[p]], acc);
+  out[i + static_cast<size_t>(k) * n] = static_cast<float>(acc);
+}
+}  // namespace
+
+namespace {
+// Y[i + k*n] = X[i + k*n] - v[i] over the columns of an n x cols matrix.
+__global__ __launch_bounds__(256) void sub_cols_kernel(const double* __restrict__ X, const double* __restrict__ v,
+                                                       double* __restrict__ Y, int n, int cols) {
+  const size_t total = static_cast<size_t>(n) * cols;
+  for (size_t q = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; q < total;
+       q += static_cast<size_t>(gridDim.x) * 256) {
+    const int i = static_cast<int>(q % static_cast<size_t>(n));
+    Y[q] = X[q] - v[i];
+  }
+}
+// X[i + k*n] += v[i], then X[i + k*n] = lb[i] where lb[i] == ub[i].
+__global__ __launch_bounds__(256) void add_cols_fix_kernel(double* __restrict__ X, const double* __restrict__ v,
+                                                           const double* __restrict__ lb,
+                                                           const double* __restrict__ ub, int n, int cols) {
+  const size_t total = static_cast<size_t>(n) * cols;
+  for (size_t q = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; q < total;
+       q += static_cast<size_t>(gridDim.x) * 256) {
+    const int i = static_cast<int>(q % static_cast<size_t>(n));
+    X[q] = lb[i] == ub[i] ? lb[i] : X[q] + v[i];
+  }
+}
+}  // namespace
+
+hipError_t launch_sub_cols(const double* X, const double* v, double* Y, int n, int cols, hipStream_t stream) {
+  const size_t total = static_cast<size_t>(n) * cols;
+  if (total == 0) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((total + 255) / 256, 65536));
+  hipLaunchKernelGGL(sub_cols_kernel, dim3(blocks), dim3(256), 0, stream, X, v, Y, n, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_cols_fix(double* X, const double* v, const double* lb, const double* ub, int n, int cols,
+                               hipStream_t stream) {
+  const size_t total = static_cast<size_t>(n) * cols;
+  if (total == 0) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((total + 255) / 256, 65536));
+  hipLaunchKernelGGL(add_cols_fix_kernel, dim3(blocks), dim3(256), 0, stream, X, v, lb, ub, n, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_pinv_csc(int n, int m, const int* colptr, const int* rowidx, const double* vals, const double* G,
+                           float* out, hipStream_t stream) {
+  if (n <= 0 || m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pinv_csc_kernel, dim3(static_cast<unsigned>((n + 255) / 256), static_cast<unsigned>(m)),
+                     dim3(256), 0, stream, n, m, colptr, rowidx, vals, G, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStream_t stream) {
   hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, stream, flag, P, idx, count);
   return hipGetLastError();

@@ -600,12 +600,18 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         if constexpr (kUni) rho = wave_uniform(rho);
         const bool reproj = rho > a.reprojTol && d > 0;
         if (reproj) ++sts.y;
-        if (a.mode == 2) {
+        // Mode 2 hands flagged chains to the batched projection round after
+        // this launch. A chain that needs no projection either stops too (the
+        // rounds then stay aligned with the checks, which reduced-precision
+        // runs use to place their FP64 rounds) or, with FP64 rounds, runs on:
+        // its projection, whenever it comes, is the same column-independent
+        // FP64 product, so the results do not change.
+        if (a.mode == 2 && (reproj || !a.stop_flagged_only)) {
           if (lane == 0) a.flag[b] = reproj ? 1 : 0;
           stopped_at_check = true;
           break;
         }
-        if (reproj) {
+        if (reproj) {  // mode 1
           for (int j = lane; j < d; j += TPC) {
             double acc = 0.0;
             const double* Nr = a.Nrm + j;

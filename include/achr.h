@@ -233,6 +233,20 @@ int achr_sample_timing(achr_sampler *sampler, double *ms_total, double *ms_step,
  * reprojection mode (1 in-kernel, 2 batched). */
 int achr_sampler_schedule(const achr_sampler *sampler, int *lanes, int *reproject_mode);
 
+/* Moves every warmup point onto the set the chains sample, {x : S x = 0,
+ * x_i = lb_i for each fixed reaction (lb_i = ub_i)}: w <- s0 + N_f N_f^T
+ * (w - s0), s0 a point of that set and N_f an orthonormal basis of its
+ * directions (FP64), then recomputes the centre. ACHR directions w - c then
+ * stay in that set, so the residual checks of a run flag only what the clip
+ * to the bounds creates. The reference samples from its warmup file as read;
+ * its "%f" text leaves every point off S x = 0 by its rounding, which is
+ * what makes ACHRCuda reproject at every check (ACHRCuda.cu:325-343). */
+int achr_sampler_project_warmup(achr_sampler *sampler);
+
+/* Copies the sampler's warmup set out (n x W column major) and its W; with
+ * warmup NULL only W. */
+int achr_sampler_get_warmup(const achr_sampler *sampler, double *warmup, int *W);
+
 /* ------------------------------------------------------------- batches */
 
 /* Several samplers (one per model, any mix of reaction counts) run as one

@@ -95,8 +95,20 @@ def test_several_continues_and_refusals(setup, tmp_path):
         smp.continue_run(5)  # other check period than the run's
     smp.set_run_options(3, 7, 10)
     smp.save_state(str(tmp_path / "s.bin"))
+    # a sampler whose own schedule has other lanes resumes the saved run with
+    # the run's 64 lanes (they fix the norm's reduction tree)
+    other = make(setup, 128, 2, False)
+    other.load_state(str(tmp_path / "s.bin"))
+    other.continue_run(10)
+    longer = make(setup, 64, 2, False)
+    longer.run(P, 45, seed=8)
+    np.testing.assert_array_equal(other.points(), longer.points())
+    np.testing.assert_array_equal(other.stats(), longer.stats())
+    raw = bytearray((tmp_path / "s.bin").read_bytes())
+    raw[20:24] = (48).to_bytes(4, "little")  # header field `lanes`: no schedule has 48 threads per chain
+    (tmp_path / "s48.bin").write_bytes(bytes(raw))
     with pytest.raises(achr.AchrError):
-        make(setup, 128, 2, False).load_state(str(tmp_path / "s.bin"))  # other lane count
+        make(setup, 64, 2, False).load_state(str(tmp_path / "s48.bin"))
     with pytest.raises(achr.AchrError):
         make(setup, 64, 2, True).load_state(str(tmp_path / "s.bin"))  # bank sampler, ACHR state
     (tmp_path / "bad.bin").write_bytes(b"not a state file at all, nothing here to read")
