@@ -57,6 +57,7 @@ SIGNATURES = {
     "achr_sampler_project_warmup": (_I, [_P]),
     "achr_sampler_get_warmup": (_I, [_P, _PD, _PI]),
     "achr_sample": (_I, [_P, _I, _I, C.c_uint64, C.c_uint32, _I]),
+    "achr_sample_noalloc": (_I, [_P, _I, _I, C.c_uint64, C.c_uint32, _I, _P, _P]),
     "achr_sample_points": (_I, [_P, _PD]),
     "achr_sample_points_device": (_I, [_P, C.POINTER(_P)]),
     "achr_sample_stats": (_I, [_P, _PI32]),

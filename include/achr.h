@@ -186,6 +186,28 @@ int achr_sampler_center(const achr_sampler *sampler, double *c0);
  */
 int achr_sample(achr_sampler *sampler, int P, int S, uint64_t seed, uint32_t file, int chain0);
 
+/* achr_sample for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * no counterpart in the reference. One file of P chains x S steps as
+ * achr_sample runs it (same schedule, same Philox streams, same points and
+ * counters, bit for bit), with the chain state in a static 64 MiB device
+ * array that the library holds on each GPU it runs on until the process ends,
+ * instead of buffers it allocates. The caller passes the outputs as device
+ * memory: out (n x P doubles, reaction major) and stats (4 int32 per chain
+ * as achr_sample_stats gives them, or NULL). The call makes no hipMalloc,
+ * hipFree or other allocation, records no event and returns once the work
+ * is queued on the sampler's stream. Limits: FP64 projection rounds only
+ * (proj_prec ACHR_PROJ_FP64), no trace, and
+ *   P * (16 n + 8 max(d, 1) + 40) + 2 KiB <= 64 MiB
+ * (e.g. 39,000 chains at n = 95, 322 at n = 10,600); larger runs are refused
+ * with ACHR_E_ARG. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads, other samplers) race,
+ * so a caller must order them. The run is not the sampler's last run:
+ * achr_sample_points, _continue and _save_state keep referring to the last
+ * achr_sample. */
+int achr_sample_noalloc(achr_sampler *sampler, int P, int S, uint64_t seed, uint32_t file, int chain0, double *out,
+                        int32_t *stats);
+
 /* Continues every chain of the last run (or of a loaded state) for S_more
  * further accepted steps, as if the run had been started with S + S_more:
  * points, counters and traces equal those of the longer run bit for bit.
