@@ -503,6 +503,7 @@ void fill_args(const achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t
   a.max_misses = S->opt.max_misses;
   a.mode = S->mode;
   a.stop_flagged_only = S->mode == 2 && S->opt.proj_prec == ACHR_PROJ_FP64;
+  a.check_budget = 10;
   a.X = S->X.p;
   a.C = S->C.p;
   a.T = S->T.p;

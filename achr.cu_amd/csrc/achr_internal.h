@@ -26,7 +26,8 @@ struct StepArgs {
   double dTol, uTol, maxMinTol, reprojTol;
   int check_every, max_misses;
   int mode;              // 1: reprojection inside the kernel; 2: stop at checks (batched GEMM)
-  int stop_flagged_only; // mode 2: a chain stops only at a check that flags it; the others run on
+  int stop_flagged_only; // mode 2: a chain stops only at a check that flags it, or
+  int check_budget;      //   after check_budget checks in one launch (launch balance)
   // state, chain major
   double* X;             // P x n: x of chain b at X + b*n
   double* C;             // P x n: running centre

@@ -388,6 +388,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
   int consec = st.w;
   bool pending = st.z == kPendingCheck;
   bool stopped_at_check = false;
+  int checks_here = 0;  // checks passed in this launch
   const uint32_t chain = static_cast<uint32_t>(a.chain0 + b);
 
   // The warmup point (or bank direction) of attempt a + 1 is loaded while
@@ -603,10 +604,13 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         // Mode 2 hands flagged chains to the batched projection round after
         // this launch. A chain that needs no projection either stops too (the
         // rounds then stay aligned with the checks, which reduced-precision
-        // runs use to place their FP64 rounds) or, with FP64 rounds, runs on:
-        // its projection, whenever it comes, is the same column-independent
-        // FP64 product, so the results do not change.
-        if (a.mode == 2 && (reproj || !a.stop_flagged_only)) {
+        // runs use to place their FP64 rounds) or, with FP64 rounds, runs on
+        // for up to check_budget checks per launch: its projection, whenever
+        // it comes, is the same column-independent FP64 product, so the
+        // results do not change, and the budget keeps a launch from waiting
+        // on the few chains that never stop.
+        ++checks_here;
+        if (a.mode == 2 && (reproj || !a.stop_flagged_only || checks_here >= a.check_budget)) {
           if (lane == 0) a.flag[b] = reproj ? 1 : 0;
           stopped_at_check = true;
           break;
