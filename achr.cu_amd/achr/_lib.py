@@ -100,6 +100,7 @@ SIGNATURES = {
     "achr_sample_continue": (_I, [_P, _I]),
     "achr_sampler_save_state": (_I, [_P, C.c_char_p]),
     "achr_sampler_load_state": (_I, [_P, C.c_char_p]),
+    "achr_quotient": (_I, [_PD, _PD, _PD, C.c_long]),
     "achr_bank_reciprocal": (_I, [_PD, _PD, C.c_long]),
     "achr_gemm_f32": (_I, [_I, _I, _I, C.POINTER(C.c_float), C.c_long, C.c_long, C.POINTER(C.c_float), C.c_long,
                            C.c_long, C.POINTER(C.c_float), C.c_long, C.c_long, C.c_float, C.c_float, _I]),

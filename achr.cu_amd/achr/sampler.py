@@ -479,6 +479,17 @@ def bank_reciprocal(u) -> np.ndarray:
     return r
 
 
+def quotient(a, b) -> np.ndarray:
+    """The device's division replacement of the plain chord, elementwise a / b."""
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(b, dtype=np.float64).ravel()
+    if a.shape != b.shape:
+        raise ValueError("a and b differ in size")
+    q = np.empty_like(a)
+    check(lib().achr_quotient(_dp(a), _dp(b), _dp(q), a.size), "achr_quotient")
+    return q
+
+
 def _fp(a):
     return a.ctypes.data_as(_ctypes.POINTER(_ctypes.c_float))
 

@@ -300,8 +300,40 @@ static int sampler_create(const achr_model* M, const double* warmup, int W, cons
   std::vector<double> Nrm(static_cast<size_t>(n) * M->d);
   for (int j = 0; j < M->d; ++j)
     for (int i = 0; i < n; ++i) Nrm[static_cast<size_t>(i) * M->d + j] = M->N[static_cast<size_t>(j) * n + i];
+  // S as a sliced ELL table for the residual check of chains of 64 or more
+  // lanes: rows sorted by length (longest first), 64 rows to a slice, each
+  // slice as wide as its longest row, entries of a row in CSR order.
+  const int m = M->m;
+  std::vector<int> order(m);
+  for (int r = 0; r < m; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return M->rowptr[x + 1] - M->rowptr[x] > M->rowptr[y + 1] - M->rowptr[y];
+  });
+  const int nslices = (m + 63) / 64;
+  std::vector<int> eptr(static_cast<size_t>(nslices) + 1, 0), erow(static_cast<size_t>(std::max(nslices, 1)) * 64, -1);
+  for (int sl = 0; sl < nslices; ++sl) {
+    const int r0 = order[static_cast<size_t>(sl) * 64];
+    eptr[sl + 1] = eptr[sl] + 64 * (M->rowptr[r0 + 1] - M->rowptr[r0]);
+  }
+  std::vector<int> ecol(std::max(eptr[nslices], 1), -1);
+  std::vector<double> eval(ecol.size(), 0.0);
+  for (int sl = 0; sl < nslices; ++sl)
+    for (int l = 0; l < 64 && sl * 64 + l < m; ++l) {
+      const int r = order[static_cast<size_t>(sl) * 64 + l];
+      erow[static_cast<size_t>(sl) * 64 + l] = r;
+      for (int q = M->rowptr[r]; q < M->rowptr[r + 1]; ++q) {
+        const size_t at = static_cast<size_t>(eptr[sl]) + 64 * static_cast<size_t>(q - M->rowptr[r]) + l;
+        ecol[at] = M->colidx[q];
+        eval[at] = M->vals[q];
+      }
+    }
+  S->nslices = nslices;
   hipStream_t s0 = nullptr;
   hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = S->ell_ptr.upload(eptr.data(), eptr.size(), s0);
+  if (e == hipSuccess) e = S->ell_row.upload(erow.data(), erow.size(), s0);
+  if (e == hipSuccess) e = S->ell_col.upload(ecol.data(), ecol.size(), s0);
+  if (e == hipSuccess) e = S->ell_val.upload(eval.data(), eval.size(), s0);
   if (e == hipSuccess) e = S->warm.upload(warmup, static_cast<size_t>(n) * W, s0);
   if (e == hipSuccess) e = S->lb.upload(M->lb.data(), n, s0);
   if (e == hipSuccess) e = S->ub.upload(M->ub.data(), n, s0);
@@ -487,6 +519,11 @@ void fill_args(const achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t
   a.rowptr = S->rowptr.p;
   a.colidx = S->colidx.p;
   a.vals = S->vals.p;
+  a.nslices = S->nslices;
+  a.ell_ptr = S->ell_ptr.p;
+  a.ell_col = S->ell_col.p;
+  a.ell_val = S->ell_val.p;
+  a.ell_row = S->ell_row.p;
   a.Ncm = S->Ncm.p;
   a.Nrm = S->Nrm.p;
   a.P = P;
@@ -887,6 +924,21 @@ int achr_bank_reciprocal(const double* u, double* r, long count) {
   if (e != hipSuccess) return fail(ACHR_E_DEVICE, "reciprocal launch: %s", hipGetErrorString(e));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(r, dr.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
+int achr_quotient(const double* a, const double* b, double* q, long count) {
+  if (!a || !b || !q || count < 0) return fail(ACHR_E_ARG, "bad quotient arguments");
+  if (count == 0) return ACHR_OK;
+  const size_t nc = static_cast<size_t>(count);
+  DevBuf<double> da, db, dq;
+  HIP_TRY(da.upload(a, nc, nullptr));
+  HIP_TRY(db.upload(b, nc, nullptr));
+  HIP_TRY(dq.ensure(nc));
+  const hipError_t e = achr::launch_quotient(da.p, db.p, dq.p, nc, nullptr);
+  if (e != hipSuccess) return fail(ACHR_E_DEVICE, "quotient launch: %s", hipGetErrorString(e));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(q, dq.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
   return ACHR_OK;
 }
 

@@ -82,6 +82,9 @@ struct achr_sampler {
   std::vector<double> c0;
   achr_host::DevBuf<double> warm, lb, ub, c0d, vals, Ncm, Nrm;
   achr_host::DevBuf<int> rowptr, colidx;
+  achr_host::DevBuf<int> ell_ptr, ell_col, ell_row;  // S as a sliced ELL table (StepArgs)
+  achr_host::DevBuf<double> ell_val;
+  int nslices = 0;
   // reduced-precision projection operands (FP32 copies)
   achr_host::DevBuf<float> Sp32;      // pseudo-inverse S+, n x m column major (i contiguous)
   achr_host::DevBuf<float> N32;       // N, n x d column major

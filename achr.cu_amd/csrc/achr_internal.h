@@ -16,6 +16,15 @@ struct StepArgs {
   const int* rowptr;     // CSR of S (m rows)
   const int* colidx;
   const double* vals;
+  // The same rows as a sliced ELL table for chains of 64 or more lanes: rows
+  // sorted by length, 64 to a slice, entry e of the slice's row l at
+  // ell_ptr[s] + 64 e + l (column -1 for padding), ell_row[64 s + l] the
+  // row's index in S (-1 past m).
+  int nslices;
+  const int* ell_ptr;
+  const int* ell_col;
+  const double* ell_val;
+  const int* ell_row;
   const double* Ncm;     // n x d column major: N[i + j*n]
   const double* Nrm;     // n x d row major:    N[i*d + j]
   // run
@@ -97,6 +106,8 @@ hipError_t launch_pinv_csc(int n, int m, const int* colptr, const int* rowidx, c
                            float* out, hipStream_t stream);
 // Elementwise FP64 -> FP32 copy of `count` values.
 hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_t stream);
+// q[i] = a[i] / b[i] by the plain chord's reciprocal-and-correction (device arrays).
+hipError_t launch_quotient(const double* a, const double* b, double* q, size_t count, hipStream_t stream);
 // r[i] = the bank-mode chord's reciprocal of u[i] (device arrays).
 hipError_t launch_bank_reciprocal(const double* u, double* r, size_t count, hipStream_t stream);
 

@@ -265,6 +265,26 @@ __device__ __forceinline__ double rcp_rn(double b) {
   return fma(y, e, y);
 }
 
+// a / b rounded to nearest, given y = rcp_rn(b): q = a y is within an ulp of
+// a / b, the residual a - b q is exact through the FMA, and q + r y rounds to
+// the correctly rounded quotient (Markstein's final correction, P.
+// Markstein, IA-64 and Elementary Functions, 2000, ch. 8), for |b| in
+// [2^-30, 2^30] and quotients and residuals clear of the subnormal range. A
+// zero numerator returns a y, which carries IEEE division's signed zero.
+// Checked against IEEE division by achr_quotient's test.
+__device__ __forceinline__ double div_rn(double a, double b, double y) {
+  const double q = a * y;
+  const double r = fma(-q, b, a);
+  const double qc = fma(r, y, q);
+  return a == 0.0 ? q : qc;
+}
+// The same without the zero case, for numerators that are never zero when
+// the quotient is used (chord distances beyond dTol).
+__device__ __forceinline__ double div_rn_nz(double a, double b, double y) {
+  const double q = a * y;
+  return fma(fma(-q, b, a), y, q);
+}
+
 __device__ __forceinline__ void bstore(rsrc_t r, int off, double v) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned int, v), r,
                                         off, 0, 0);
@@ -482,12 +502,23 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         ss = group_sum<TPC>(ss, red + kRedSum * kRedSlots);
         if constexpr (kUni) ss = wave_uniform(ss);
         nrm = sqrt(ss);
+        // u / |u| (the reference divides each component, ACHRCuda.cu:264-267):
+        // one correctly rounded reciprocal of the norm and a corrected
+        // product per component give the same bits as the divisions; norms
+        // outside [2^-30, 2^30] divide.
+        if (nrm >= 0x1p-30 && nrm <= 0x1p30) {
+          const double yn = rcp_rn(nrm);
+#pragma unroll
+          for (int k = 0; k < E; ++k) u[k] = div_rn(u[k], nrm, yn);
+        } else {
+#pragma unroll
+          for (int k = 0; k < E; ++k) u[k] = u[k] / nrm;
+        }
       }
       double mn = -INFINITY, mx = INFINITY;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const double ui = DIR ? u[k] : u[k] / nrm;
-        u[k] = ui;
+        const double ui = u[k];
         if (kLook > 0 && k + kLook < E && k + kLook >= BP::lds) bq[k + kLook] = bounds(k + kLook);
         const double2 bk = (kLook > 0 && k >= BP::lds) ? bq[k] : bounds(k);
         const double lbi = bk.x, ubi = bk.y;
@@ -503,17 +534,19 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         const double dl = x[k] - lbi;
         // Both branches of the reference (ACHRCuda.cu:270-279) divide the same
         // two distances by u_i; divide once and select, so a wavefront whose
-        // lanes disagree on the sign does not run both sets of divisions. A
-        // bank direction (no reference counterpart) takes one reciprocal of
-        // u_i and two products instead of two divisions.
+        // lanes disagree on the sign does not run both sets of divisions.
+        // Here the two quotients share one correctly rounded reciprocal of
+        // u_i and are corrected to the IEEE quotients (|u_i| > uTol >= 2^-30
+        // and both distances > dTol wherever the chord uses them). A bank
+        // direction (no reference counterpart) takes the plain products.
         double r_lo, r_hi;
+        const double rcp = rcp_rn(ui);
         if constexpr (DIR) {
-          const double rcp = rcp_rn(ui);
           r_lo = -dl * rcp;
           r_hi = du * rcp;
         } else {
-          r_lo = -dl / ui;
-          r_hi = du / ui;
+          r_lo = div_rn_nz(-dl, ui, rcp);
+          r_hi = div_rn_nz(du, ui, rcp);
         }
         // A kept coordinate has both distances > dTol > 0, so the two
         // quotients have opposite signs: the lower candidate is their
@@ -572,6 +605,41 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         group_sync<TPC>();
         double rho = 0.0;
         float* rrow = (a.mode == 2 && a.R32) ? a.R32 + static_cast<size_t>(b) * a.m : nullptr;
+        if constexpr (TPC >= 64) {
+          // Sliced ELL: wavefront v of the chain takes slices v, v + TPC/64,
+          // ...; lane l sums row l of the slice. The trip count is the
+          // slice's width for every lane and the entry loads are one
+          // coalesced line per wavefront, with no row pointer to wait for;
+          // each row's products are added in CSR order, as above.
+          constexpr int NW = TPC / 64;
+          const int l = lane & 63;
+          for (int sl = __builtin_amdgcn_readfirstlane(lane >> 6); sl < a.nslices; sl += NW) {
+            const int e0 = __builtin_amdgcn_readfirstlane(a.ell_ptr[sl]);
+            const int width = (__builtin_amdgcn_readfirstlane(a.ell_ptr[sl + 1]) - e0) >> 6;
+            double acc = 0.0;
+            for (int e = 0; e < width; e += 4) {
+              int ci[4];
+              double vv[4], xv[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const int at = e0 + (min(e + q, width - 1) << 6) + l;
+                ci[q] = a.ell_col[at];
+                vv[q] = a.ell_val[at];
+              }
+#pragma unroll
+              for (int q = 0; q < 4; ++q) xv[q] = xrow.ld(ci[q] < 0 ? 0 : ci[q]);
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                if (e + q < width && ci[q] >= 0) acc = fma(vv[q], xv[q], acc);
+            }
+            const int row = a.ell_row[(sl << 6) + l];
+            if (row >= 0) {
+              const double av = fabs(acc);
+              if (av > rho) rho = av;
+              if (rrow) rrow[row] = static_cast<float>(acc);
+            }
+          }
+        } else
         for (int j = lane; j < a.m; j += TPC) {
           double acc = 0.0;
           const int p0 = a.rowptr[j], p1 = a.rowptr[j + 1];
@@ -810,6 +878,22 @@ __global__ __launch_bounds__(256) void bank_reciprocal_kernel(const double* __re
     r[i] = rcp_rn(u[i]);
 }
 }  // namespace
+
+namespace {
+__global__ __launch_bounds__(256) void quotient_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                      double* __restrict__ q, size_t count) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < count;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    q[i] = div_rn(a[i], b[i], rcp_rn(b[i]));
+}
+}  // namespace
+
+hipError_t launch_quotient(const double* a, const double* b, double* q, size_t count, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((count + 255) / 256, 65536);
+  hipLaunchKernelGGL(quotient_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, a, b, q, count);
+  return hipGetLastError();
+}
 
 hipError_t launch_bank_reciprocal(const double* u, double* r, size_t count, hipStream_t stream) {
   if (count == 0) return hipSuccess;

@@ -13,7 +13,8 @@ agreement, split-R-hat). "ecoli" is the 72 x 95 fixture model, 500 chains x
 (aggregate chain-steps/s); "precision" the projection rounds in FP64, FP32
 and split-BF16x3 on 65,536 chains; "convergence" short runs against a long
 plain run; "warmup" the 21,200 flux-variability LPs of the genome model's
-warmup set (LPs converged per second).
+warmup set (LPs converged per second); "putida" the reference's own P. putida
+model through both command lines, end to end, as its paper times it.
 
 Besides the plain FP64 run from the warmup as read (the reference's
 behaviour: a "%f" warmup is off S x = 0 by its rounding, so every check
@@ -709,12 +710,96 @@ def gate(out, q, plain_qs):
     return g
 
 
+def run_putida(args):
+    """The reference's own model end to end through the two command lines, as
+    its paper times it (paper/paper.md:71-73: wall time with the null space,
+    1,000 steps per point): createWarmupPts P_Putida.mps 2120 (the 2n flux-
+    variability LPs on the GPU), then ACHRCuda P_Putida.mps <warmup> 1 1000
+    1000 and 1 10000 1000 (--lossless for bit-exact output). Reports each
+    run's wall time and chain-steps/s, the drift |S x| of the samples, and a
+    16-chain file checked bit for bit against the oracle (with the null
+    space the library computes, handed to both)."""
+    import subprocess
+    import tempfile
+    import achr
+    import scipy.sparse as sp
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    mps = os.path.join(ROOT, "tests", "golden", "P_Putida.mps")
+    work = tempfile.mkdtemp(prefix="putida_")
+    model = achr.Model.read_mps(mps)
+    n, m_, nnz, _ = model.dims
+    rp, ci, va = model.csr()
+    lb, ub = model.bounds()
+    Ssp = sp.csr_matrix((va, ci, rp), shape=(m_, n))
+    src = os.path.join(work, "P_Putida.mps")
+    with open(mps, "rb") as f, open(src, "wb") as g:
+        g.write(f.read())
+    t = time.perf_counter()
+    r = subprocess.run([achr.WARMUP_CLI_PATH, src, str(2 * n)], cwd=work, capture_output=True, text=True)
+    t_warm = time.perf_counter() - t
+    if r.returncode:
+        raise RuntimeError(r.stderr)
+    conv_line = [l for l in r.stdout.splitlines() if "converged" in l]
+    log(f"createWarmupPts: {t_warm:.1f}s; {conv_line}")
+    wcsv = os.path.join(work, f"P_Putida{2 * n}warmup.csv")
+    runs = {}
+    for P in (1000, 10000):
+        t = time.perf_counter()
+        r = subprocess.run([achr.CLI_PATH, src, wcsv, "1", str(P), "1000", "--lossless"], cwd=work,
+                           capture_output=True, text=True)
+        wall = time.perf_counter() - t
+        if r.returncode:
+            raise RuntimeError(r.stderr)
+        pts = achr.read_points_bin(os.path.join(work, "File0.bin"))
+        lanes_line = [l for l in r.stdout.splitlines() if "threads per chain" in l]
+        runs[P] = {"points": P, "steps": 1000, "wall_s": wall, "chain_steps_per_s": P * 1000 / wall,
+                   "drift_inf": float(np.abs(Ssp @ pts).max()),
+                   "in_bounds": bool((pts >= lb[:, None] - 1e-12).all() and (pts <= ub[:, None] + 1e-12).all()),
+                   "cli": lanes_line[0] if lanes_line else ""}
+        log(f"ACHRCuda 1 x {P} x 1000: {runs[P]}")
+    # 16 chains against the oracle, the null space handed to both
+    model.compute_nullspace()
+    N = model.nullspace()
+    npath = os.path.join(work, "N.bin")
+    np.asfortranarray(N).ravel(order="F").tofile(npath)
+    r = subprocess.run([achr.CLI_PATH, src, wcsv, "1", "16", "1000", "--lossless"], cwd=work,
+                       env=dict(os.environ, ACHR_NULLSPACE=npath), capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(r.stderr)
+    g = achr.read_points_bin(os.path.join(work, "File0.bin"))
+    lanes = int([l for l in r.stdout.splitlines() if "threads per chain" in l][0].split(",")[1].split()[0])
+    Wc = np.loadtxt(wcsv, delimiter=",")
+    t = time.perf_counter()
+    o, ost, _ = O.sample(warm=Wc, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=m_, P=16, S=1000, seed=0,
+                         lanes=lanes, threads=min(16, os.cpu_count() or 1))
+    t_cpu = time.perf_counter() - t
+    parity = {"chains": 16, "steps": 1000, "lanes": lanes, "bit_equal": bool(np.array_equal(g, o)),
+              "max_abs_diff": float(np.abs(g - o).max()), "reprojections_per_chain": float(ost[:, 1].mean())}
+    log(f"oracle parity: {parity}")
+    big = runs[10000]
+    out = {"n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "the reference's P. putida model (VFWarmup/lib/P_Putida.mps)",
+           "metric": "chain-steps/s, ACHRCuda end to end (null space + sampling + File0), P. putida",
+           "value": big["chain_steps_per_s"], "unit": "chain-steps/s", "ms_per_step": big["wall_s"] * 1e3,
+           "config": {"workload": f"putida: {m_} x {n} model, createWarmupPts {2 * n} then ACHRCuda 1 x P x 1000",
+                      "reactions": n, "metabolites": m_},
+           "warmup_generation": {"lps": 2 * n, "wall_s": t_warm, "report": conv_line},
+           "runs": {str(k): v for k, v in runs.items()}, "oracle_parity": parity,
+           "paper_context": "Tesla K40, 30,000 warmup points: 17 s (1,000 points) and 138 s (10,000 points), "
+                            "paper/paper.md:71-73; other hardware, context only",
+           "cpu_baseline": {"value": 16 * 1000 / t_cpu, "unit": "chain-steps/s",
+                            "cores": min(16, os.cpu_count() or 1), "kind": "port",
+                            "sample": f"oracle (C, OpenMP) 16 chains x 1,000 steps in {t_cpu:.1f}s"}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed files (bench steps)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed files (short: 20 chain steps)")
-    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision", "convergence", "warmup"],
+    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision", "convergence", "warmup", "putida"],
                     default="genome")
     ap.add_argument("--warmup-iters", type=int, default=8000, help="warmup workload: PDHG iteration cap")
     ap.add_argument("--cpu-lps", type=int, default=1, help="warmup workload: LPs HiGHS times (about 1 min each)")
@@ -763,6 +848,9 @@ def main():
         return
     if args.workload == "warmup":
         run_warmup(args)
+        return
+    if args.workload == "putida":
+        run_putida(args)
         return
     if args.workload == "batch":
         if args.cpu_chains == 32 and args.cpu_steps == 300:

@@ -462,6 +462,14 @@ int achr_gemm_f32(int M, int N, int K, const float *A, long sa_m, long sa_k, con
  * equal IEEE division for every |u| in [2^-30, 2^30]; tests hold it to that. */
 int achr_bank_reciprocal(const double *u, double *r, long count);
 
+/* The quotient the plain chord and the direction's normalisation compute on
+ * the device in place of a division, q[i] = a[i] / b[i] for `count` host
+ * values (copied to and from the device): the reciprocal above and one
+ * corrected product. It must equal IEEE division, signed zeros included,
+ * for |b| in [2^-30, 2^30] and quotients in the normal range; tests hold it
+ * to that. */
+int achr_quotient(const double *a, const double *b, double *q, long count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,3 +110,34 @@ def test_bank_reciprocal_is_ieee_division():
     r = achr.bank_reciprocal(u)
     bad = np.flatnonzero(r.view(np.uint64) != (1.0 / u).view(np.uint64))
     assert bad.size == 0, f"{bad.size} misrounded, first u = {u[bad[:4]]!r}"
+
+
+def test_chord_quotient_is_ieee_division():
+    """The plain sampler's division replacement (a correctly rounded
+    reciprocal and one corrected product) against IEEE division, bit for bit
+    with signed zeros: 4M quotients with |b| in [2^-30, 2^30] (the norm and
+    every |u_i| the chord uses) and |a| in [2^-45, 2^15] or zero, delicate
+    significands of b included."""
+    rng = np.random.default_rng(12)
+    count = 1 << 22
+
+    def f64(sign, exp, sig):
+        return ((sign << np.uint64(63)) | (exp << np.uint64(52)) | sig).view(np.float64)
+
+    sig = rng.integers(0, 1 << 52, count, dtype=np.uint64)
+    special = np.array([(1 << 52) - 1, 0, 1, 1 << 51, (1 << 52) - 2, (1 << 51) - 1, (1 << 51) + 1, 2, 3],
+                       dtype=np.uint64)
+    sig[:special.size * 61] = np.repeat(special, 61)
+    exp = rng.integers(1023 - 30, 1023 + 30, count, dtype=np.uint64)
+    exp[:special.size * 61] = np.tile(np.arange(1023 - 30, 1023 + 31, dtype=np.uint64), special.size)
+    b = f64(rng.integers(0, 2, count, dtype=np.uint64), exp, sig)
+    asig = rng.integers(0, 1 << 52, count, dtype=np.uint64)
+    asig[count // 2: count // 2 + special.size * 61] = np.repeat(special, 61)
+    a = f64(rng.integers(0, 2, count, dtype=np.uint64), rng.integers(1023 - 45, 1023 + 15, count, dtype=np.uint64),
+            asig)
+    a[:1000] = 0.0
+    a[1000:2000] = -0.0
+    q = achr.quotient(a, b)
+    want = a / b
+    bad = np.flatnonzero(q.view(np.uint64) != want.view(np.uint64))
+    assert bad.size == 0, f"{bad.size} misrounded, first a = {a[bad[:3]]!r}, b = {b[bad[:3]]!r}"
