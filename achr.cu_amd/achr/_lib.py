@@ -101,6 +101,7 @@ SIGNATURES = {
     "achr_sampler_save_state": (_I, [_P, C.c_char_p]),
     "achr_sampler_load_state": (_I, [_P, C.c_char_p]),
     "achr_quotient": (_I, [_PD, _PD, _PD, C.c_long]),
+    "achr_lp_polish": (_I, [_I, _I, _PD, _PD, _PD, _PD, _PD, _PD, _I]),
     "achr_bank_reciprocal": (_I, [_PD, _PD, C.c_long]),
     "achr_gemm_f32": (_I, [_I, _I, _I, C.POINTER(C.c_float), C.c_long, C.c_long, C.POINTER(C.c_float), C.c_long,
                            C.c_long, C.POINTER(C.c_float), C.c_long, C.c_long, C.c_float, C.c_float, _I]),
@@ -135,6 +136,7 @@ class AchrWarmupOptions(C.Structure):
         ("check_every", C.c_int),
         ("tol", C.c_double),
         ("seed", C.c_uint64),
+        ("polish", C.c_int),
     ]
 
 

@@ -387,7 +387,7 @@ def write_points_csv(path: str, points: np.ndarray) -> None:
 
 
 def warmup_points(model: "Model", nPts: int, max_iters: int | None = None, tol: float | None = None,
-                  seed: int | None = None, check_every: int | None = None):
+                  seed: int | None = None, check_every: int | None = None, polish: bool | None = None):
     """Warmup set by flux variability + random-objective LPs on the GPU
     (achr_warmup_generate). Returns (W: n x nPts, one point per column;
     fva: n x 2 minimum / maximum per reaction; converged: nPts bools, one per
@@ -402,15 +402,18 @@ def warmup_points(model: "Model", nPts: int, max_iters: int | None = None, tol: 
         o.seed = seed
     if check_every is not None:
         o.check_every = check_every
+    if polish is not None:
+        o.polish = 1 if polish else 0
     n = model.n
     W = np.empty((n, nPts), order="F")
     fva = np.empty((2, n))
-    rep = np.empty(4)
+    rep = np.empty(6)
     conv = np.empty(nPts, dtype=np.int32)
     check(lib().achr_warmup_generate(model._h, nPts, C.byref(o), W.ctypes.data_as(C.POINTER(C.c_double)), _dp(fva),
                                      conv.ctypes.data_as(C.POINTER(C.c_int)), _dp(rep)), "achr_warmup_generate")
     return W, fva.T.copy(), conv.astype(bool), {"iterations": int(rep[0]), "converged": int(rep[1]),
-                                                "ms": float(rep[2]), "scaled_norm": float(rep[3])}
+                                                "ms": float(rep[2]), "scaled_norm": float(rep[3]),
+                                                "polished": int(rep[4]), "polish_ms": float(rep[5])}
 
 
 def write_points_bin(path: str, points: np.ndarray) -> None:
@@ -488,6 +491,17 @@ def quotient(a, b) -> np.ndarray:
     q = np.empty_like(a)
     check(lib().achr_quotient(_dp(a), _dp(b), _dp(q), a.size), "achr_quotient")
     return q
+
+
+def lp_polish(N, lb, ub, c, xfeas, x0, max_iters: int = 0):
+    """achr_lp_polish: min c^T x over {x = N z, lb <= x <= ub}; returns (status, x)."""
+    N = np.asfortranarray(N, dtype=np.float64)
+    n, d = N.shape
+    flat = np.ascontiguousarray(N.ravel(order="F")) if d else np.zeros(1)
+    x = np.array(x0, dtype=np.float64)
+    args = [np.ascontiguousarray(v, dtype=np.float64) for v in (lb, ub, c, xfeas)]
+    rc = lib().achr_lp_polish(n, d, _dp(flat), *[_dp(v) for v in args], _dp(x), max_iters or 50 * d + 1000)
+    return rc, x
 
 
 def _fp(a):

@@ -19,12 +19,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "../../include/achr.h"
 #include "achr_host.h"
 #include "achr_internal.h"
 #include "achr_philox.h"
+#include "achr_polish.h"
 
 using achr_host::DevBuf;
 using achr_host::fail;
@@ -282,6 +284,7 @@ void achr_warmup_options_default(achr_warmup_options* o) {
   o->check_every = 64;
   o->tol = 1e-9;
   o->seed = 1;
+  o->polish = 1;
 }
 
 int achr_warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* opts, double* warmup, double* fva,
@@ -292,6 +295,69 @@ int achr_warmup_generate(const achr_model* M, int nPts, const achr_warmup_option
 }  // extern "C"
 
 namespace {
+
+// The LPs the iteration left unconverged (done[b] == 0), finished exactly
+// on the host (achr_polish.cpp) in the coordinates of an orthonormal null-space
+// basis of S: from the model's own, or one computed for the purpose when the
+// model has none. Feasible reference point: the mean of the converged
+// solutions (of all, when none converged). xs: n x B, reaction major.
+int polish_unconverged(const achr_model* M, int nf, int R, const std::vector<double>& Cr, std::vector<double>& xs,
+                       std::vector<int>& done, int* polished, double* ms) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = M->n, B = static_cast<int>(done.size());
+  std::vector<double> N;
+  int d = M->d;
+  if (d >= 0) {
+    N = M->N;
+  } else {
+    achr_model tmp = *M;
+    const int rc = achr_model_compute_nullspace(&tmp, &d);
+    if (rc != ACHR_OK) return rc;
+    N.swap(tmp.N);
+  }
+  *polished = 0;
+  if (static_cast<double>(n) * d > 3e7) return ACHR_OK;  // beyond the host method's reach
+  std::vector<double> Nr(static_cast<size_t>(n) * std::max(d, 1));
+  for (int j = 0; j < d; ++j)
+    for (int i = 0; i < n; ++i) Nr[static_cast<size_t>(i) * d + j] = N[static_cast<size_t>(j) * n + i];
+  std::vector<int> todo;
+  int nconv = 0;
+  for (int b = 0; b < B; ++b) {
+    if (done[b]) ++nconv;
+    else todo.push_back(b);
+  }
+  std::vector<double> xf(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const double* row = xs.data() + static_cast<size_t>(i) * B;
+    double s = 0.0;
+    for (int b = 0; b < B; ++b)
+      if (done[b] || nconv == 0) s += row[b];
+    xf[i] = s / (nconv ? nconv : B);
+  }
+  std::vector<int> ok(todo.size(), 0);
+  const unsigned hw = std::thread::hardware_concurrency();
+  achr_host::polish_many(static_cast<int>(todo.size()), static_cast<int>(std::min(hw ? hw : 4u, 16u)), [&](int q) {
+    const int b = todo[q];
+    std::vector<double> c(n, 0.0), x(n);
+    const double sgn = (b & 1) ? -1.0 : 1.0;
+    if (b < 2 * nf) c[b >> 1] = sgn;
+    else
+      for (int i = 0; i < nf; ++i) c[i] = sgn * Cr[static_cast<size_t>(i) * R + (b - 2 * nf)];
+    for (int i = 0; i < n; ++i) x[i] = xs[static_cast<size_t>(i) * B + b];
+    if (achr_host::polish_lp(n, d, Nr.data(), M->lb.data(), M->ub.data(), c.data(), xf.data(), x.data(),
+                             50 * d + 1000) == 0) {
+      for (int i = 0; i < n; ++i) xs[static_cast<size_t>(i) * B + b] = x[i];
+      ok[q] = 1;
+    }
+  });
+  for (size_t q = 0; q < todo.size(); ++q)
+    if (ok[q]) {
+      done[todo[q]] = 1;
+      ++*polished;
+    }
+  *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return ACHR_OK;
+}
 
 int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* opts, double* warmup, double* fva,
                     int* converged, double* report) {
@@ -487,9 +553,9 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
   // points: unscale, clip to the bounds (movePtsBds), FVA columns, then the
   // move toward the centre x <- 0.33 x + 0.67 c (createWarmupPtsGLPK.c:36-65)
   std::vector<double> xs(nB);
+  std::vector<int> done(B);
   HIP_TRY(hipMemcpy(xs.data(), X.p, sizeof(double) * nB, hipMemcpyDeviceToHost));
-  if (converged) HIP_TRY(hipMemcpy(converged, d_done.p, sizeof(int) * B, hipMemcpyDeviceToHost));
-  std::vector<double> cen(n, 0.0);
+  HIP_TRY(hipMemcpy(done.data(), d_done.p, sizeof(int) * B, hipMemcpyDeviceToHost));
   for (int i = 0; i < n; ++i) {
     double* row = xs.data() + static_cast<size_t>(i) * B;
     for (int b = 0; b < B; ++b) {
@@ -498,6 +564,18 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
       else if (v > M->ub[i]) v = M->ub[i];
       row[b] = v;
     }
+  }
+  int polished = 0;
+  double polish_ms = 0.0;
+  if (o.polish && ndone < B) {
+    const int rc = polish_unconverged(M, nf, R, Cr, xs, done, &polished, &polish_ms);
+    if (rc != ACHR_OK) return rc;
+    ndone += polished;
+  }
+  if (converged) std::copy(done.begin(), done.end(), converged);
+  std::vector<double> cen(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const double* row = xs.data() + static_cast<size_t>(i) * B;
     if (fva) {
       fva[i] = i < nf ? row[2 * i] : NAN;
       fva[static_cast<size_t>(n) + i] = i < nf ? row[2 * i + 1] : NAN;
@@ -515,6 +593,8 @@ int warmup_generate(const achr_model* M, int nPts, const achr_warmup_options* op
     report[1] = static_cast<double>(ndone);
     report[2] = solve_ms;
     report[3] = nrm;
+    report[4] = static_cast<double>(polished);
+    report[5] = polish_ms;
   }
   return ACHR_OK;
 }

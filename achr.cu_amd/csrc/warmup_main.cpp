@@ -13,7 +13,8 @@
 // as GLPK does for the reference (an E / L / G row is S_i x = / <= / >= b_i);
 // for the usual S x = 0 model that is the sampler's own set. Environment:
 // ACHR_SEED (random objectives), ACHR_WARMUP_ITERS (iteration cap, default
-// 40,000), ACHR_WARMUP_TOL (default 1e-9).
+// 40,000), ACHR_WARMUP_TOL (default 1e-9), ACHR_WARMUP_POLISH (0 leaves LPs
+// unconverged at the cap as they stopped; default 1 finishes them exactly).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -59,12 +60,15 @@ int main(int argc, char** argv) {
   if (const char* s = std::getenv("ACHR_SEED")) opt.seed = std::strtoull(s, nullptr, 10);
   if (const char* s = std::getenv("ACHR_WARMUP_ITERS")) opt.max_iters = std::atoi(s);
   if (const char* s = std::getenv("ACHR_WARMUP_TOL")) opt.tol = std::atof(s);
-  std::vector<double> W(static_cast<size_t>(n) * nPts), fva(2 * static_cast<size_t>(n)), rep(4);
+  if (const char* s = std::getenv("ACHR_WARMUP_POLISH")) opt.polish = std::atoi(s);
+  std::vector<double> W(static_cast<size_t>(n) * nPts), fva(2 * static_cast<size_t>(n)), rep(6);
   std::vector<int> conv(nPts);
   if (achr_warmup_generate(model, static_cast<int>(nPts), &opt, W.data(), fva.data(), conv.data(), rep.data()) !=
       ACHR_OK)
     return die("solving the LPs");
-  std::printf("%.0f of %ld LPs converged in %.0f iterations, %.3f s\n", rep[1], nPts, rep[0], rep[2] * 1e-3);
+  std::printf("%.0f of %ld LPs converged in %.0f iterations, %.3f s", rep[1], nPts, rep[0], rep[2] * 1e-3);
+  if (rep[4] > 0) std::printf(" (%.0f finished by the exact polish, %.3f s)", rep[4], rep[5] * 1e-3);
+  std::printf("\n");
   if (rep[1] < nPts)
     std::printf("Warning: %ld LPs stopped at the iteration cap; their points are approximate "
                 "(raise ACHR_WARMUP_ITERS)\n", nPts - static_cast<long>(rep[1]));

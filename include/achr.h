@@ -402,6 +402,9 @@ typedef struct {
   double tol;       /* per LP, on the rescaled problem: |A x| and the duality gap relative to
                        1 + |objectives| (1e-9) */
   uint64_t seed;    /* random objectives (1) */
+  int polish;       /* 1 (default): LPs still unconverged at max_iters are finished exactly on the host
+                       by a primal active-set method in null-space coordinates (models with n x d up to
+                       3e7); 0: left as the solver stopped */
 } achr_warmup_options;
 
 void achr_warmup_options_default(achr_warmup_options *opts);
@@ -419,8 +422,9 @@ void achr_warmup_options_default(achr_warmup_options *opts);
  * reference's LPs) and fva holds NaN for the auxiliary columns. converged (nPts ints, may be NULL): 1 for each LP
  * whose iterate met tol, 0 for one stopped by max_iters (a first-order method:
  * its value is approximate, its point only approximately on S x = 0).
- * report (4 doubles, may be NULL): iterations, LPs converged, solve
- * milliseconds, |A~|_2 of the rescaled matrix. The reference solves one LP at
+ * report (6 doubles, may be NULL): iterations, LPs converged (after the
+ * polish), solve milliseconds, |A~|_2 of the rescaled matrix, LPs finished by
+ * the polish, polish milliseconds. The reference solves one LP at
  * a time with GLPK's simplex over MPI ranks. */
 int achr_warmup_generate(const achr_model *model, int nPts, const achr_warmup_options *opts, double *warmup,
                          double *fva, int *converged, double *report);
@@ -469,6 +473,17 @@ int achr_bank_reciprocal(const double *u, double *r, long count);
  * for |b| in [2^-30, 2^30] and quotients in the normal range; tests hold it
  * to that. */
 int achr_quotient(const double *a, const double *b, double *q, long count);
+
+/* The exact finish the warmup generator applies to LPs its first-order
+ * iteration leaves unconverged: min c^T x s.t. x = N z, lb <= x <= ub, with N
+ * (n x d, column major) an orthonormal basis of null(S), by a primal
+ * active-set method in z on the host. x: in, a start; out, the solution.
+ * xfeas: a point within the bounds on S x = 0 toward which an infeasible start
+ * is pulled. Returns 0 at a KKT point, 1 at the iteration cap, 2 without a
+ * feasible start, 3 for an unbounded direction, -1 for bad arguments, -7
+ * out of memory. */
+int achr_lp_polish(int n, int d, const double *N, const double *lb, const double *ub, const double *c,
+                   const double *xfeas, double *x, int max_iters);
 
 #ifdef __cplusplus
 }

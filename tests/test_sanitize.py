@@ -202,6 +202,19 @@ DRIVER = textwrap.dedent(r'''
     O.sample(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S.shape[0], P=3, S=40, seed=1, lanes=16,
              bank=bank, c0=O.center(We))
     O.philox((1, 2, 3, 4), (5, 6))
+    # the warmup generator's exact LP finish (host code), on the 72 x 95 model
+    L.achr_lp_polish.argtypes = [C.c_int, C.c_int, PD, PD, PD, PD, PD, PD, C.c_int]
+    Nf = np.ascontiguousarray(N.ravel(order="F"))
+    xf = O.center(We)
+    for i in range(0, S.shape[1], 7):
+        for sgn in (1.0, -1.0):
+            c = np.zeros(S.shape[1])
+            c[i] = sgn
+            x = xf + 3.0 * np.cos(np.arange(S.shape[1]))
+            rc = L.achr_lp_polish(S.shape[1], N.shape[1], Nf.ctypes.data_as(PD), lb.ctypes.data_as(PD),
+                                  ub.ctypes.data_as(PD), c.ctypes.data_as(PD), xf.ctypes.data_as(PD),
+                                  x.ctypes.data_as(PD), 5000)
+            note("lp_polish", rc)
     print("SANITIZED OK", counts, flush=True)
 ''')
 
