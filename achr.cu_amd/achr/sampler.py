@@ -474,6 +474,27 @@ def gemm_f32(A, B, prec: int = 0, alpha: float = 1.0, C=None, beta: float = 0.0,
     return out
 
 
+def gemm_sub_f64(A, B, C, idx=None, prec: int = 0) -> np.ndarray:
+    """The residual projection round's kernel: a copy of C (FP64, rows x N)
+    with rows idx (all when None) reduced by A[idx] @ B (A rows x K, B K x N,
+    FP32 operands, prec 0 FP32 MFMA, 1 split-BF16x3)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    out = np.array(C, dtype=np.float64, order="C")
+    rows, K = A.shape
+    K2, N = B.shape
+    if K2 != K or out.shape != (rows, N):
+        raise ValueError("shapes differ")
+    iv = None if idx is None else np.ascontiguousarray(idx, dtype=np.int32)
+    M = rows if iv is None else int(iv.size)
+    F = _ctypes.POINTER(_ctypes.c_float)
+    check(lib().achr_gemm_sub_f64(rows, M, N, K, A.ctypes.data_as(F), B.ctypes.data_as(F),
+                                  out.ctypes.data_as(_ctypes.POINTER(_ctypes.c_double)),
+                                  None if iv is None else iv.ctypes.data_as(_ctypes.POINTER(_ctypes.c_int)), prec),
+          "achr_gemm_sub_f64")
+    return out
+
+
 def bank_reciprocal(u) -> np.ndarray:
     """The device reciprocal of the direction-bank chord, elementwise."""
     u = np.ascontiguousarray(u, dtype=np.float64).ravel()

@@ -352,8 +352,16 @@ static int sampler_create(const achr_model* M, const double* warmup, int W, cons
   if (e == hipSuccess) e = hipEventCreate(&S->ev_end);
   if (e != hipSuccess) return fail(ACHR_E_DEVICE, "sampler upload: %s", hipGetErrorString(e));
   if (o.proj_prec == ACHR_PROJ_RES_BF16X3 || o.proj_prec == ACHR_PROJ_RES_F32) {
-    const int rc = build_pinv_f32(M, S->Sp32);
+    DevBuf<float> Sp32;  // S+, n x m column major (i contiguous)
+    const int rc = build_pinv_f32(M, Sp32);
     if (rc != ACHR_OK) return rc;
+    // B(k, i) = S+(i, k) for the rounds' X -= R S+^T, packed once
+    const int gp = o.proj_prec == ACHR_PROJ_RES_F32 ? achr::kPrecF32 : achr::kPrecBF16x3;
+    S->r32_ld = (std::max(M->m, 1) + 15) / 16 * 16;
+    e = S->Bpk.ensure(std::max<size_t>(achr::packed_b_bytes(M->m, n), 1));
+    if (e == hipSuccess) e = achr::launch_pack_b(Sp32.p, n, 1, M->m, n, gp, S->Bpk.p, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(ACHR_E_DEVICE, "packing S+: %s", hipGetErrorString(e));
   }
   *out = S.release();
   return ACHR_OK;
@@ -480,7 +488,11 @@ int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   const int prec = S->opt.proj_prec;
   const bool res_form = prec == ACHR_PROJ_RES_BF16X3 || prec == ACHR_PROJ_RES_F32;
   const bool null_low = prec == ACHR_PROJ_NULL_F32 || prec == ACHR_PROJ_NULL_BF16X3;
-  if (res_form) HIP_TRY(S->R32.ensure(static_cast<size_t>(P) * std::max(S->m, 1)));
+  if (res_form) {
+    // residual rows, zero in the padding columns the GEMM reads past m
+    HIP_TRY(S->R32.ensure(static_cast<size_t>(P) * S->r32_ld));
+    HIP_TRY(hipMemsetAsync(S->R32.p, 0, sizeof(float) * P * S->r32_ld, st));
+  }
   if (null_low) HIP_TRY(S->T32.ensure(static_cast<size_t>(P) * std::max(S->d, 1)));
   S->rounds_fp64 = 0;
   S->rounds_low = 0;
@@ -549,6 +561,7 @@ void fill_args(const achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t
   a.flag = S->flag.p;
   const int prec = S->opt.proj_prec;
   a.R32 = (prec == ACHR_PROJ_RES_BF16X3 || prec == ACHR_PROJ_RES_F32) ? S->R32.p : nullptr;
+  a.r32_ld = S->r32_ld;
   a.out = S->out.p;
   a.trace = S->trace_len > 0 ? S->trace.p : nullptr;
   a.trace_chains = tr_chains;
@@ -575,26 +588,20 @@ int enqueue_projection(achr_sampler* S, int kind, hipStream_t st) {
   if (le == hipSuccess && fp64_round) {
     le = achr::launch_project(S->Ncm.p, n, S->d, S->X.p, S->T.p, S->idx.p, S->count.p, P, st);
   } else if (le == hipSuccess && res_form) {
-    // X(q, :) -= sum_k R(q, k) S+(:, k): A = residual rows, B(k, i) = S+[i + k*n]
-    achr::GemmArgs g{};
+    // X(q, :) -= sum_k R(q, k) S+(:, k): A = residual rows, B = S+^T packed
+    achr::GemmSubArgs g{};
     g.M = P;
     g.N = n;
-    g.K = S->m;
+    g.Kp = S->r32_ld;
     g.Mdev = S->count.p;
     g.A = S->R32.p;
-    g.sa_m = S->m;
-    g.sa_k = 1;
+    g.lda = S->r32_ld;
     g.idxA = S->idx.p;
-    g.B = S->Sp32.p;
-    g.sb_k = n;
-    g.sb_n = 1;
+    g.B = S->Bpk.p;
     g.C = S->X.p;
-    g.sc_m = n;
-    g.sc_n = 1;
+    g.ldc = n;
     g.idxC = S->idx.p;
-    g.alpha = 1.0f;
-    g.epi = achr::kEpiSubF64;
-    le = achr::launch_gemm(g, prec == ACHR_PROJ_RES_F32 ? achr::kPrecF32 : achr::kPrecBF16x3, P, st);
+    le = achr::launch_gemm_sub(g, prec == ACHR_PROJ_RES_F32 ? achr::kPrecF32 : achr::kPrecBF16x3, P, st);
   } else if (le == hipSuccess) {
     const int gp = prec == ACHR_PROJ_NULL_F32 ? achr::kPrecF32 : achr::kPrecBF16x3;
     // T(q, j) = sum_i X(q, i) N(i, j)
@@ -974,6 +981,49 @@ int achr_gemm_f32(int M, int N, int K, const float* A, long sa_m, long sa_k, con
   if (e != hipSuccess) return fail(ACHR_E_ARG, "gemm launch: %s", hipGetErrorString(e));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(C, dC.p, sizeof(float) * nc, hipMemcpyDeviceToHost));
+  return ACHR_OK;
+}
+
+int achr_gemm_sub_f64(int rows, int M, int N, int K, const float* A, const float* B, double* C, const int* idx,
+                      int prec) {
+  if (rows <= 0 || M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return fail(ACHR_E_ARG, "bad GEMM arguments");
+  if (prec != 0 && prec != 1) return fail(ACHR_E_ARG, "prec %d: 0 (FP32) or 1 (split BF16x3)", prec);
+  if (!idx && M > rows) return fail(ACHR_E_ARG, "M=%d rows of %d", M, rows);
+  if (idx)
+    for (int q = 0; q < M; ++q)
+      if (idx[q] < 0 || idx[q] >= rows) return fail(ACHR_E_ARG, "idx[%d]=%d outside %d rows", q, idx[q], rows);
+  const int Kp = (K + 15) / 16 * 16;
+  std::vector<float> Ap(static_cast<size_t>(rows) * Kp, 0.0f);
+  for (int r = 0; r < rows; ++r)
+    std::copy(A + static_cast<size_t>(r) * K, A + static_cast<size_t>(r) * K + K, Ap.begin() + static_cast<size_t>(r) * Kp);
+  DevBuf<float> dA, dB;
+  DevBuf<unsigned char> dP;
+  DevBuf<double> dC;
+  DevBuf<int> dI, dM;
+  HIP_TRY(dA.upload(Ap.data(), Ap.size(), nullptr));
+  HIP_TRY(dB.upload(B, static_cast<size_t>(K) * N, nullptr));
+  HIP_TRY(dC.upload(C, static_cast<size_t>(rows) * N, nullptr));
+  HIP_TRY(dM.upload(&M, 1, nullptr));
+  if (idx) HIP_TRY(dI.upload(idx, M, nullptr));
+  HIP_TRY(dP.ensure(achr::packed_b_bytes(K, N)));
+  const int gp = prec == 0 ? achr::kPrecF32 : achr::kPrecBF16x3;
+  hipError_t e = achr::launch_pack_b(dB.p, N, 1, K, N, gp, dP.p, nullptr);
+  achr::GemmSubArgs g{};
+  g.M = M;
+  g.N = N;
+  g.Kp = Kp;
+  g.Mdev = dM.p;
+  g.A = dA.p;
+  g.lda = Kp;
+  g.idxA = idx ? dI.p : nullptr;
+  g.B = dP.p;
+  g.C = dC.p;
+  g.ldc = N;
+  g.idxC = idx ? dI.p : nullptr;
+  if (e == hipSuccess) e = achr::launch_gemm_sub(g, gp, M, nullptr);
+  if (e != hipSuccess) return fail(ACHR_E_DEVICE, "gemm_sub launch: %s", hipGetErrorString(e));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(C, dC.p, sizeof(double) * rows * N, hipMemcpyDeviceToHost));
   return ACHR_OK;
 }
 

@@ -86,9 +86,10 @@ struct achr_sampler {
   achr_host::DevBuf<double> ell_val;
   int nslices = 0;
   // reduced-precision projection operands (FP32 copies)
-  achr_host::DevBuf<float> Sp32;      // pseudo-inverse S+, n x m column major (i contiguous)
+  achr_host::DevBuf<unsigned char> Bpk;  // pseudo-inverse S+ packed for the residual rounds (launch_pack_b)
+  int r32_ld = 0;                        // row stride of R32: m rounded up to 16, zero beyond m
   achr_host::DevBuf<float> N32;       // N, n x d column major
-  achr_host::DevBuf<float> R32, T32;  // per-chain residuals (P x m) and N^T x (P x d)
+  achr_host::DevBuf<float> R32, T32;  // per-chain residuals (P x r32_ld) and N^T x (P x d)
   int rounds_fp64 = 0, rounds_low = 0;
   // direction bank (learned sampler)
   achr_host::DevBuf<double> dirs;     // n x K column major, unit columns

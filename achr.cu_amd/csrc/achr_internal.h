@@ -44,7 +44,8 @@ struct StepArgs {
   int4* st;              // {attempt, s, phase, consecutive misses}
   int4* stats;           // {misses, reprojections, checks, flags}
   int* flag;             // batched mode: 1 if chain b asks for a reprojection this round
-  float* R32;            // batched mode, optional: residual S x of each check, P x m
+  float* R32;            // batched mode, optional: residual S x of each check, row b at R32 + b*r32_ld
+  long r32_ld;
   double* out;           // n x P reaction major
   int dir_mode;          // 1: warm holds a bank of unit directions (no centre)
   double* trace;

@@ -278,6 +278,15 @@ __device__ __forceinline__ double div_rn(double a, double b, double y) {
   const double qc = fma(r, y, q);
   return a == 0.0 ? q : qc;
 }
+// The same for b > 0 without a select: the residual is formed with the
+// opposite sign, s = q b - a = -(a - q b) exactly, and the correction is
+// q - s y. For a = +-0, q = a y carries a's sign, s = +0 and q - s y adds a
+// zero of q's sign, so the result is a's signed zero as IEEE division gives;
+// otherwise every step equals div_rn's (rounding to nearest is symmetric).
+__device__ __forceinline__ double div_rn_pos(double a, double b, double y) {
+  const double q = a * y;
+  return fma(fma(q, b, -a), -y, q);
+}
 // The same without the zero case, for numerators that are never zero when
 // the quotient is used (chord distances beyond dTol).
 __device__ __forceinline__ double div_rn_nz(double a, double b, double y) {
@@ -454,6 +463,9 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
   for (;;) {
     if (!pending) {
       if (s >= a.S) break;
+      // the step length's uniform, drawn ahead (the scalar unit forms it
+      // while the vector unit works on the direction and the chord)
+      const double rs = stream_u01(a.seed, static_cast<uint32_t>(s), chain, a.file, a.model, kTagStep);
       double u[E];
       double ss = 0.0;
       if constexpr (kPre) {
@@ -509,7 +521,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         if (nrm >= 0x1p-30 && nrm <= 0x1p30) {
           const double yn = rcp_rn(nrm);
 #pragma unroll
-          for (int k = 0; k < E; ++k) u[k] = div_rn(u[k], nrm, yn);
+          for (int k = 0; k < E; ++k) u[k] = div_rn_pos(u[k], nrm, yn);
         } else {
 #pragma unroll
           for (int k = 0; k < E; ++k) u[k] = u[k] / nrm;
@@ -587,7 +599,6 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         continue;
       }
       consec = 0;
-      const double rs = stream_u01(a.seed, static_cast<uint32_t>(s), chain, a.file, a.model, kTagStep);
       const double t = fma(rs, mx - mn, mn);
 #pragma unroll
       for (int k = 0; k < E; ++k) x[k] = fma(t, u[k], x[k]);
@@ -604,7 +615,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         for (int k = 0; k < E; ++k) xrow.st(lane + k * TPC, x[k]);
         group_sync<TPC>();
         double rho = 0.0;
-        float* rrow = (a.mode == 2 && a.R32) ? a.R32 + static_cast<size_t>(b) * a.m : nullptr;
+        float* rrow = (a.mode == 2 && a.R32) ? a.R32 + static_cast<size_t>(b) * a.r32_ld : nullptr;
         if constexpr (TPC >= 64) {
           // Sliced ELL: wavefront v of the chain takes slices v, v + TPC/64,
           // ...; lane l sums row l of the slice. The trip count is the
@@ -884,7 +895,7 @@ __global__ __launch_bounds__(256) void quotient_kernel(const double* __restrict_
                                                       double* __restrict__ q, size_t count) {
   for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < count;
        i += static_cast<size_t>(gridDim.x) * blockDim.x)
-    q[i] = div_rn(a[i], b[i], rcp_rn(b[i]));
+    q[i] = b[i] > 0.0 ? div_rn_pos(a[i], b[i], rcp_rn(b[i])) : div_rn(a[i], b[i], rcp_rn(b[i]));
 }
 }  // namespace
 

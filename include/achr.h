@@ -461,6 +461,17 @@ void achr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t o
 int achr_gemm_f32(int M, int N, int K, const float *A, long sa_m, long sa_k, const float *B, long sb_k, long sb_n,
                   float *C, long sc_m, long sc_n, float alpha, float beta, int prec);
 
+/* The residual form of the reduced-precision projection rounds on host
+ * arrays (copied to and from the device): for q < M, row r = idx[q] (or q
+ * when idx is NULL) of the FP64 rows x N matrix C becomes
+ * C[r, :] - A[r, :] B, with A rows x K and B K x N row-major FP32, in the
+ * kernel the sampler's rounds use (B packed once, 256 x 256 tiles, FP32
+ * accumulation). prec: 0 FP32 MFMA, 1 split-BF16x3. No reference
+ * counterpart: the reference reprojects in FP64 (ACHRcu/ACHRCuda.cu:338-343);
+ * tests hold this kernel to an FP64 product. */
+int achr_gemm_sub_f64(int rows, int M, int N, int K, const float *A, const float *B, double *C, const int *idx,
+                      int prec);
+
 /* The reciprocal the direction-bank chord computes on the device, r[i] =
  * 1/u[i], for `count` host values (copied to and from the device). It must
  * equal IEEE division for every |u| in [2^-30, 2^30]; tests hold it to that. */

@@ -25,6 +25,7 @@ import scipy.sparse as sp
 import achr
 import oracle as O
 from achr import synth
+from test_gpu_precision import check_sub
 
 pytestmark = pytest.mark.gpu
 
@@ -158,6 +159,26 @@ def test_projection_gemm_at_genome_sizes(prec, form):
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     say(f"relative error {err:.3g}")
     assert err < GEMM_TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_residual_round_kernel_at_genome_sizes(prec):
+    """The kernel of the residual rounds (packed S+, 256 x 256 tiles) at the
+    genome model's sizes: 700 of 1,000 residual rows, gathered in random
+    order, each updated as C[r] -= A[r] B with K = 5,800 and N = 10,600
+    (neither a multiple of the tile), against the FP64 product of the same
+    FP32 operands; the other rows are left bit for bit."""
+    rng = np.random.default_rng(17 + prec)
+    rows, M, K, N = 1000, 700, 5800, 10600
+    A = rng.standard_normal((rows, K)).astype(np.float32)
+    B = (rng.standard_normal((K, N)) * 1e-2).astype(np.float32)
+    C0 = rng.standard_normal((rows, N))
+    idx = rng.permutation(rows)[:M].astype(np.int32)
+    say(f"gemm_sub prec={prec} {M} of {rows} rows, {N}x{K}")
+    got = achr.gemm_sub_f64(A, B, C0, idx=idx, prec=prec)
+    check_sub(A[idx], B, C0[idx] - got[idx], prec)
+    rest = np.setdiff1d(np.arange(rows), idx)
+    assert np.array_equal(got[rest], C0[rest])
 
 
 def test_reduced_precision_rounds_on_genome_model(genome):

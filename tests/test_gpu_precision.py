@@ -35,6 +35,52 @@ def test_gemm_matches_numpy(prec, shape, ta, tb):
     assert err < TOL[prec], err
 
 
+# Bounds for the residual rounds' kernel. Elementwise, guaranteed: FP32
+# accumulation of K products (3K for the three split passes) costs at most
+# about (terms + 2) 2^-24 of sum_k |a_k b_k|; the split a = hi + bf16(a - hi)
+# represents each operand to 2^-16 of itself and the dropped lo * lo term is
+# below 2^-16 |a b|, so the split adds 3 2^-16 of the same sum. A wrong
+# pairing of k, a dropped pass or a transposed operand misses by O(1).
+# Frobenius, statistical (Gaussian data): about 5e-6 for the split (its
+# representation error does not average out over k), ~1e-7 for FP32.
+SUB_TOL = {0: 2e-6, 1: 1e-5}
+
+
+def check_sub(A, B, upd, prec):
+    """upd (the FP64 change of the rows) against A @ B in FP64."""
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    ref = A64 @ B64
+    K = A.shape[1]
+    mag = np.abs(A64) @ np.abs(B64)
+    terms = K if prec == 0 else 3 * K
+    bound = ((terms + 2) * 2.0 ** -24 + (0 if prec == 0 else 3 * 2.0 ** -16)) * mag + 1e-300
+    worst = float((np.abs(upd - ref) / bound).max())
+    err = np.linalg.norm(upd - ref) / np.linalg.norm(ref)
+    print(f"relative error {err:.3g}, worst elementwise {worst:.3g} of its bound", flush=True)
+    assert worst <= 1.0 and err < SUB_TOL[prec], (worst, err)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("shape", [(300, 300, 37, 1), (300, 257, 600, 100), (1000, 700, 513, 33),
+                                   (40, 3, 256, 16), (513, 512, 1, 2000)])
+def test_residual_round_kernel_matches_numpy(prec, shape):
+    """The residual rounds' kernel C[r] -= A[r] B on packed B: ragged tiles
+    in every dimension (K from 1, N from 1, partial row tiles), rows gathered
+    in random order when M < rows, the rest left bit for bit."""
+    rows, M, N, K = shape
+    print(f"gemm_sub prec={prec} rows={rows} M={M} N={N} K={K}", flush=True)
+    rng = np.random.default_rng(rows + M + N + K)
+    A = rng.standard_normal((rows, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    C0 = rng.standard_normal((rows, N))
+    idx = None if M == rows else rng.permutation(rows)[:M].astype(np.int32)
+    got = achr.gemm_sub_f64(A, B, C0, idx=idx, prec=prec)
+    sel = np.arange(rows) if idx is None else idx
+    check_sub(A[sel], B, C0[sel] - got[sel], prec)
+    rest = np.setdiff1d(np.arange(rows), sel)
+    assert np.array_equal(got[rest], C0[rest])
+
+
 @pytest.fixture(scope="module")
 def c1_models(c1):
     S, lb, ub, Wf, We = c1
