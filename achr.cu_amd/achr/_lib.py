@@ -104,7 +104,7 @@ SIGNATURES = {
     "achr_lp_polish": (_I, [_I, _I, _PD, _PD, _PD, _PD, _PD, _PD, _I]),
     "achr_bank_reciprocal": (_I, [_PD, _PD, C.c_long]),
     "achr_gemm_sub_f64": (_I, [_I, _I, _I, _I, C.POINTER(C.c_float), C.POINTER(C.c_float), _PD, C.POINTER(C.c_int),
-                               _I]),
+                               _I, C.POINTER(C.c_float)]),
     "achr_gemm_f32": (_I, [_I, _I, _I, C.POINTER(C.c_float), C.c_long, C.c_long, C.POINTER(C.c_float), C.c_long,
                            C.c_long, C.POINTER(C.c_float), C.c_long, C.c_long, C.c_float, C.c_float, _I]),
 }

@@ -474,10 +474,11 @@ def gemm_f32(A, B, prec: int = 0, alpha: float = 1.0, C=None, beta: float = 0.0,
     return out
 
 
-def gemm_sub_f64(A, B, C, idx=None, prec: int = 0) -> np.ndarray:
+def gemm_sub_f64(A, B, C, idx=None, prec: int = 0, timing: bool = False):
     """The residual projection round's kernel: a copy of C (FP64, rows x N)
     with rows idx (all when None) reduced by A[idx] @ B (A rows x K, B K x N,
-    FP32 operands, prec 0 FP32 MFMA, 1 split-BF16x3)."""
+    FP32 operands, prec 0 FP32 MFMA, 1 split-BF16x3). With timing, returns
+    (result, kernel milliseconds)."""
     A = np.ascontiguousarray(A, dtype=np.float32)
     B = np.ascontiguousarray(B, dtype=np.float32)
     out = np.array(C, dtype=np.float64, order="C")
@@ -488,11 +489,13 @@ def gemm_sub_f64(A, B, C, idx=None, prec: int = 0) -> np.ndarray:
     iv = None if idx is None else np.ascontiguousarray(idx, dtype=np.int32)
     M = rows if iv is None else int(iv.size)
     F = _ctypes.POINTER(_ctypes.c_float)
+    ms = _ctypes.c_float(0.0)
     check(lib().achr_gemm_sub_f64(rows, M, N, K, A.ctypes.data_as(F), B.ctypes.data_as(F),
                                   out.ctypes.data_as(_ctypes.POINTER(_ctypes.c_double)),
-                                  None if iv is None else iv.ctypes.data_as(_ctypes.POINTER(_ctypes.c_int)), prec),
+                                  None if iv is None else iv.ctypes.data_as(_ctypes.POINTER(_ctypes.c_int)), prec,
+                                  _ctypes.byref(ms) if timing else None),
           "achr_gemm_sub_f64")
-    return out
+    return (out, float(ms.value)) if timing else out
 
 
 def bank_reciprocal(u) -> np.ndarray:

@@ -357,7 +357,7 @@ static int sampler_create(const achr_model* M, const double* warmup, int W, cons
     if (rc != ACHR_OK) return rc;
     // B(k, i) = S+(i, k) for the rounds' X -= R S+^T, packed once
     const int gp = o.proj_prec == ACHR_PROJ_RES_F32 ? achr::kPrecF32 : achr::kPrecBF16x3;
-    S->r32_ld = (std::max(M->m, 1) + 15) / 16 * 16;
+    S->r32_ld = achr::packed_k(std::max(M->m, 1));
     e = S->Bpk.ensure(std::max<size_t>(achr::packed_b_bytes(M->m, n), 1));
     if (e == hipSuccess) e = achr::launch_pack_b(Sp32.p, n, 1, M->m, n, gp, S->Bpk.p, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -985,14 +985,14 @@ int achr_gemm_f32(int M, int N, int K, const float* A, long sa_m, long sa_k, con
 }
 
 int achr_gemm_sub_f64(int rows, int M, int N, int K, const float* A, const float* B, double* C, const int* idx,
-                      int prec) {
+                      int prec, float* kernel_ms) {
   if (rows <= 0 || M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return fail(ACHR_E_ARG, "bad GEMM arguments");
   if (prec != 0 && prec != 1) return fail(ACHR_E_ARG, "prec %d: 0 (FP32) or 1 (split BF16x3)", prec);
   if (!idx && M > rows) return fail(ACHR_E_ARG, "M=%d rows of %d", M, rows);
   if (idx)
     for (int q = 0; q < M; ++q)
       if (idx[q] < 0 || idx[q] >= rows) return fail(ACHR_E_ARG, "idx[%d]=%d outside %d rows", q, idx[q], rows);
-  const int Kp = (K + 15) / 16 * 16;
+  const int Kp = achr::packed_k(K);
   std::vector<float> Ap(static_cast<size_t>(rows) * Kp, 0.0f);
   for (int r = 0; r < rows; ++r)
     std::copy(A + static_cast<size_t>(r) * K, A + static_cast<size_t>(r) * K + K, Ap.begin() + static_cast<size_t>(r) * Kp);
@@ -1020,9 +1020,17 @@ int achr_gemm_sub_f64(int rows, int M, int N, int K, const float* A, const float
   g.C = dC.p;
   g.ldc = N;
   g.idxC = idx ? dI.p : nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess && kernel_ms) e = hipEventCreate(&t0);
+  if (e == hipSuccess && kernel_ms) e = hipEventCreate(&t1);
+  if (e == hipSuccess && kernel_ms) e = hipEventRecord(t0, nullptr);
   if (e == hipSuccess) e = achr::launch_gemm_sub(g, gp, M, nullptr);
-  if (e != hipSuccess) return fail(ACHR_E_DEVICE, "gemm_sub launch: %s", hipGetErrorString(e));
-  HIP_TRY(hipDeviceSynchronize());
+  if (e == hipSuccess && kernel_ms) e = hipEventRecord(t1, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess && kernel_ms) e = hipEventElapsedTime(kernel_ms, t0, t1);
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  if (e != hipSuccess) return fail(ACHR_E_DEVICE, "gemm_sub: %s", hipGetErrorString(e));
   HIP_TRY(hipMemcpy(C, dC.p, sizeof(double) * rows * N, hipMemcpyDeviceToHost));
   return ACHR_OK;
 }

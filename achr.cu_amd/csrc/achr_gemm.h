@@ -38,7 +38,7 @@ hipError_t launch_gemm(const GemmArgs& g, int prec, int max_rows, hipStream_t s)
 // The projection rounds' residual form on packed operands:
 //   C[rowC(i), j] -= sum_k A[rowA(i) * lda + k] B(k, j)  (i < M, j < N)
 // with FP32 accumulation (kPrecF32 or kPrecBF16x3). A is FP32, k contiguous,
-// lda a multiple of 4 and columns K..Kp-1 zero (Kp = K rounded up to 16);
+// lda a multiple of 4 and columns K..Kp-1 zero (Kp = packed_k(K));
 // B is packed once by launch_pack_b for the same precision; C is FP64.
 // rowA(i) = idxA ? idxA[i] : i, likewise rowC. M is *Mdev when Mdev is set,
 // with max_rows bounding the launch.
@@ -53,11 +53,13 @@ struct GemmSubArgs {
   long ldc;
   const int* idxC;
 };
+// The reduction length gemm_sub runs over: K rounded up to its slab depth.
+int packed_k(int K);
 // Bytes of B packed for a K x N operand (either precision).
 size_t packed_b_bytes(int K, int N);
 // Packs B(k, j) = src[k * sk + j * sj] (K x N, FP32) for gemm_sub: 256-column
-// tiles of 16-deep slabs, [tile][slab][column][k], zero beyond K and N; FP32
-// values, or per slab the BF16 high parts followed by the BF16 low parts.
+// tiles of slabs, [tile][slab][column][k], zero beyond K and N; FP32 values,
+// or per slab the BF16 high parts followed by the BF16 low parts.
 hipError_t launch_pack_b(const float* src, long sk, long sj, int K, int N, int prec, void* dst, hipStream_t s);
 hipError_t launch_gemm_sub(const GemmSubArgs& g, int prec, int max_rows, hipStream_t s);
 

@@ -516,6 +516,8 @@ hipError_t launch_sub(const GemmSubArgs& g, int max_rows, hipStream_t s) {
 
 }  // namespace
 
+int packed_k(int K) { return (K + GK - 1) / GK * GK; }
+
 size_t packed_b_bytes(int K, int N) {
   const size_t nt = (static_cast<size_t>(N) + GT - 1) / GT, nslab = (static_cast<size_t>(K) + GK - 1) / GK;
   return nt * nslab * GT * GK * 4;

@@ -87,7 +87,7 @@ struct achr_sampler {
   int nslices = 0;
   // reduced-precision projection operands (FP32 copies)
   achr_host::DevBuf<unsigned char> Bpk;  // pseudo-inverse S+ packed for the residual rounds (launch_pack_b)
-  int r32_ld = 0;                        // row stride of R32: m rounded up to 16, zero beyond m
+  int r32_ld = 0;                        // row stride of R32: packed_k(m), zero beyond m
   achr_host::DevBuf<float> N32;       // N, n x d column major
   achr_host::DevBuf<float> R32, T32;  // per-chain residuals (P x r32_ld) and N^T x (P x d)
   int rounds_fp64 = 0, rounds_low = 0;

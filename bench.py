@@ -11,7 +11,8 @@ plain FP64 sampler runs the same file for comparison (speedup, moment
 agreement, split-R-hat). "ecoli" is the 72 x 95 fixture model, 500 chains x
 1,000 steps; "batch" is 64 models of 100..13,000 reactions in one job
 (aggregate chain-steps/s); "precision" the projection rounds in FP64, FP32
-and split-BF16x3 on 65,536 chains; "convergence" short runs against a long
+and split-BF16x3 on 65,536 chains; "gemm" the residual round's GEMM alone at
+that shape (kernel TFLOP/s against the MFMA peak); "convergence" short runs against a long
 plain run; "warmup" the 21,200 flux-variability LPs of the genome model's
 warmup set (LPs converged per second); "putida" the reference's own P. putida
 model through both command lines, end to end, as its paper times it.
@@ -540,6 +541,46 @@ def run_precision(args):
     print(json.dumps(out), flush=True)
 
 
+def run_gemm(args):
+    """The residual projection round's GEMM alone, at the precision
+    workload's shape: 65,536 residual rows over K = 5,800 metabolites times
+    the packed S+ (N = 10,600 reactions), subtracted from FP64 chains, on
+    synthetic operands, in FP32 MFMA and split-BF16x3. Kernel time from HIP
+    events on the kernel's stream (packing and host copies excluded). No
+    solver runs in the process, so counter runs can profile it."""
+    import achr
+    P = args.chains or 65536
+    K, N = 5800, 10600
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((P, K), dtype=np.float32)
+    B = rng.standard_normal((K, N), dtype=np.float32) * np.float32(1e-2)
+    C = np.zeros((P, N))
+    flop = 2.0 * P * K * N
+    res = {}
+    for prec, name, passes, peak in ((0, "f32", 1, 157.3), (1, "bf16x3", 3, 2500.0)):
+        times = []
+        for _ in range(args.warmup + args.steps):
+            _, ms = achr.gemm_sub_f64(A, B, C, prec=prec, timing=True)
+            times.append(ms)
+        ms = float(np.median(times[args.warmup:]))
+        tf = flop / (ms * 1e-3) / 1e12
+        res[name] = {"kernel_ms": ms, "launch_ms": times, "useful_tflops": tf, "mfma_tflops": tf * passes,
+                     "mfma_peak_tflops": peak, "frac": tf * passes / peak}
+        log(f"gemm_sub {name}: {res[name]}")
+    b = res["bf16x3"]
+    out = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 (split-bf16x3 MFMA)", "data": "synthetic",
+           "metric": "TFLOP/s of the residual projection round's GEMM (useful work, split-BF16x3)",
+           "value": b["useful_tflops"], "unit": "TFLOP/s", "ms_per_step": b["kernel_ms"],
+           "config": {"workload": f"gemm: {P} residual rows x {K} metabolites x {N} reactions, FP64 update",
+                      "rows": P, "K": K, "N": N},
+           "roofline": {"bound": "mfma", "kernel": "gemm_sub_kernel<split-BF16x3>", "achieved": b["mfma_tflops"],
+                        "peak": 2500.0, "unit": "TFLOP/s", "frac": b["frac"], "traffic": None,
+                        "per_launch": "3 BF16 MFMA passes x 2 P K N flop"},
+           "variants": res}
+    print(json.dumps(out), flush=True)
+
+
 def run_convergence(args):
     """Moment error of plain and learned ACHR against a long plain run on the
     genome model: the reference is plain ACHR with --ref-steps steps (residual
@@ -804,7 +845,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1, help="timed files (bench steps)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed files (short: 20 chain steps)")
-    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision", "convergence", "warmup", "putida"],
+    ap.add_argument("--workload", choices=["genome", "ecoli", "batch", "precision", "gemm", "convergence", "warmup",
+                                               "putida"],
                     default="genome")
     ap.add_argument("--warmup-iters", type=int, default=8000, help="warmup workload: PDHG iteration cap")
     ap.add_argument("--cpu-lps", type=int, default=1, help="warmup workload: LPs HiGHS times (about 1 min each)")
@@ -847,6 +889,9 @@ def main():
         calibrate_copy(args.calibrate_copy_mib)
     if args.workload == "precision":
         run_precision(args)
+        return
+    if args.workload == "gemm":
+        run_gemm(args)
         return
     if args.workload == "convergence":
         run_convergence(args)

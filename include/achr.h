@@ -466,11 +466,12 @@ int achr_gemm_f32(int M, int N, int K, const float *A, long sa_m, long sa_k, con
  * when idx is NULL) of the FP64 rows x N matrix C becomes
  * C[r, :] - A[r, :] B, with A rows x K and B K x N row-major FP32, in the
  * kernel the sampler's rounds use (B packed once, 256 x 256 tiles, FP32
- * accumulation). prec: 0 FP32 MFMA, 1 split-BF16x3. No reference
- * counterpart: the reference reprojects in FP64 (ACHRcu/ACHRCuda.cu:338-343);
- * tests hold this kernel to an FP64 product. */
+ * accumulation). prec: 0 FP32 MFMA, 1 split-BF16x3. kernel_ms, if not NULL,
+ * receives the kernel's duration (HIP events; packing and copies excluded).
+ * No reference counterpart: the reference reprojects in FP64
+ * (ACHRcu/ACHRCuda.cu:338-343); tests hold this kernel to an FP64 product. */
 int achr_gemm_sub_f64(int rows, int M, int N, int K, const float *A, const float *B, double *C, const int *idx,
-                      int prec);
+                      int prec, float *kernel_ms);
 
 /* The reciprocal the direction-bank chord computes on the device, r[i] =
  * 1/u[i], for `count` host values (copied to and from the device). It must
