@@ -63,6 +63,7 @@ SIGNATURES = {
     "achr_sample_stats": (_I, [_P, _PI32]),
     "achr_sample_trace": (_I, [_P, _PD]),
     "achr_sample_timing": (_I, [_P, _PD, _PD, _PD, _PI, _PI]),
+    "achr_sample_timing_check": (_I, [_P, _PD, _PI]),
     "achr_sampler_schedule": (_I, [_P, _PI, _PI]),
     "achr_write_points_csv": (_I, [C.c_char_p, _PD, _I, _I]),
     "achr_write_points_bin": (_I, [C.c_char_p, _PD, _I, _I]),
@@ -127,6 +128,7 @@ class AchrOptions(C.Structure):
         ("proj_prec", C.c_int),
         ("fp64_every", C.c_int),
         ("model_id", C.c_int),
+        ("check_kernel", C.c_int),
     ]
 
 

@@ -162,11 +162,12 @@ class Options:
     proj_prec: int = 0
     fp64_every: int = 10
     model_id: int = 0
+    check_kernel: int = 0
 
     def to_c(self) -> AchrOptions:
         return AchrOptions(self.dTol, self.uTol, self.maxMinTol, self.reprojTol, self.check_every,
                            self.max_misses, self.lanes, self.reproject_mode, self.trace_chains, self.trace_thin,
-                           self.proj_prec, self.fp64_every, self.model_id)
+                           self.proj_prec, self.fp64_every, self.model_id, self.check_kernel)
 
 
 PROJ_FP64, PROJ_RES_BF16X3, PROJ_RES_F32, PROJ_NULL_F32, PROJ_NULL_BF16X3 = range(5)
@@ -272,8 +273,11 @@ class Sampler:
         ns, np_ = C.c_int(), C.c_int()
         check(lib().achr_sample_timing(self._h, C.byref(tot), C.byref(stp), C.byref(prj), C.byref(ns),
                                        C.byref(np_)), "achr_sample_timing")
+        chk, nc = C.c_double(), C.c_int()
+        check(lib().achr_sample_timing_check(self._h, C.byref(chk), C.byref(nc)), "achr_sample_timing_check")
         return {"ms_total": tot.value, "ms_step": stp.value, "ms_proj": prj.value,
-                "step_launches": ns.value, "proj_launches": np_.value}
+                "step_launches": ns.value, "proj_launches": np_.value,
+                "ms_check": chk.value, "check_launches": nc.value}
 
     def sample(self, P: int, S: int, seed: int = 0, file: int = 0, chain0: int = 0) -> np.ndarray:
         self.run(P, S, seed, file, chain0)

@@ -53,6 +53,7 @@ void achr_options_default(achr_options* o) {
   o->proj_prec = ACHR_PROJ_FP64;
   o->fp64_every = 10;
   o->model_id = 0;
+  o->check_kernel = 0;
 }
 
 
@@ -260,6 +261,7 @@ static int sampler_create(const achr_model* M, const double* warmup, int W, cons
   achr_options_default(&o);
   if (opts) o = *opts;
   if (o.check_every <= 0 || o.max_misses < 0) return fail(ACHR_E_ARG, "bad check_every / max_misses");
+  if (o.check_kernel < 0 || o.check_kernel > 2) return fail(ACHR_E_ARG, "bad check_kernel=%d", o.check_kernel);
   if (o.lanes != 0 && !achr::schedule_supported(o.lanes, achr::elems_for(M->n, o.lanes)))
     return fail(ACHR_E_ARG, "lanes=%d cannot hold n=%d reactions", o.lanes, M->n);
   if (o.trace_chains > 0 && o.trace_thin <= 0) return fail(ACHR_E_ARG, "trace_thin must be > 0");
@@ -505,6 +507,7 @@ int prepare_run(achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t file
   HIP_TRY(hipMemsetAsync(S->flag.p, 0, sizeof(int) * P, st));
   S->ev_step.clear();
   S->ev_proj.clear();
+  S->ev_check.clear();
   S->seed = seed;
   S->file = file;
   S->chain0 = chain0;
@@ -648,7 +651,39 @@ int enqueue_projection(achr_sampler* S, int kind, hipStream_t st) {
   return ACHR_OK;
 }
 
-int enqueue_segments(achr_sampler* S, const achr::StepArgs& a, int segments, int checks_done, hipStream_t st) {
+bool use_check_kernel(const achr_sampler* S, const achr::Schedule& sc) {
+  // plain chains only: a direction bank keeps fixed reactions fixed, its
+  // checks rarely flag and run every 100 steps, and it has no kernel without
+  // the in-kernel check
+  if (S->mode != 2 || S->d <= 0 || S->use_dirs || S->n > achr::kMaxCheckReactions || S->opt.check_kernel == 1)
+    return false;
+  return S->opt.check_kernel == 2 || sc.tpc >= 512;
+}
+
+achr::CheckArgs check_args(const achr_sampler* S, const achr::StepArgs& a) {
+  achr::CheckArgs c{};
+  c.n = a.n;
+  c.d = a.d;
+  c.P = a.P;
+  c.nslices = a.nslices;
+  c.ell_ptr = a.ell_ptr;
+  c.ell_col = a.ell_col;
+  c.ell_val = a.ell_val;
+  c.ell_row = a.ell_row;
+  c.X = a.X;
+  c.st = a.st;
+  c.stats = a.stats;
+  c.flag = a.flag;
+  c.R32 = a.R32;
+  c.r32_ld = a.r32_ld;
+  c.reprojTol = a.reprojTol;
+  (void)S;
+  return c;
+}
+
+int enqueue_segments(achr_sampler* S, const achr::StepArgs& args, int segments, int checks_done, hipStream_t st) {
+  achr::StepArgs a = args;
+  a.check_out = use_check_kernel(S, S->run_sched) ? 1 : 0;
   size_t evn = 0;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventRecord(S->ev_begin, st));
@@ -662,6 +697,17 @@ int enqueue_segments(achr_sampler* S, const achr::StepArgs& a, int segments, int
     HIP_TRY(hipEventRecord(e1, st));
     S->ev_step.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
     evn += 2;
+    if (a.check_out && seg + 1 < segments) {
+      rc = event_at(S, evn, &e0);
+      if (rc == ACHR_OK) rc = event_at(S, evn + 1, &e1);
+      if (rc != ACHR_OK) return rc;
+      HIP_TRY(hipEventRecord(e0, st));
+      const hipError_t ce = achr::launch_check(check_args(S, a), st);
+      if (ce != hipSuccess) return fail(ACHR_E_DEVICE, "check kernel launch: %s", hipGetErrorString(ce));
+      HIP_TRY(hipEventRecord(e1, st));
+      S->ev_check.emplace_back(static_cast<int>(evn), static_cast<int>(evn + 1));
+      evn += 2;
+    }
     const int kind = seg + 1 < segments ? projection_kind(S, checks_done + seg) : kProjNone;
     if (kind != kProjNone) {
       rc = event_at(S, evn, &e0);
@@ -736,6 +782,7 @@ int achr_sample_continue(achr_sampler* S, int S_more) {
   S->have_run = false;
   S->ev_step.clear();
   S->ev_proj.clear();
+  S->ev_check.clear();
   achr::StepArgs a{};
   achr_host::fill_args(S, S->P, S1, S->seed, S->file, S->chain0, &a);
   const int ce = S->opt.check_every;
@@ -917,6 +964,21 @@ int achr_sample_timing(achr_sampler* S, double* ms_total, double* ms_step, doubl
   if (ms_proj) *ms_proj = tproj;
   if (n_step) *n_step = static_cast<int>(S->ev_step.size());
   if (n_proj) *n_proj = static_cast<int>(S->ev_proj.size());
+  return ACHR_OK;
+}
+
+int achr_sample_timing_check(achr_sampler* S, double* ms_check, int* n_check) {
+  if (!S) return fail(ACHR_E_ARG, "sampler is NULL");
+  if (!S->have_run) return fail(ACHR_E_STATE, "no run");
+  HIP_TRY(hipEventSynchronize(S->ev_end));
+  double t = 0.0;
+  for (auto& pr : S->ev_check) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, S->ev_pool[pr.first], S->ev_pool[pr.second]));
+    t += ms;
+  }
+  if (ms_check) *ms_check = t;
+  if (n_check) *n_check = static_cast<int>(S->ev_check.size());
   return ACHR_OK;
 }
 

@@ -1,0 +1,156 @@
+// Residual check of a whole segment's chains in one launch (batched rounds).
+//
+// The reference checks each chain every 10th accepted step with a COO SpMV
+// r = S x and a serial max |r| (findMaxAbs, ACHRcu/ACHRCuda.cu:232-240,
+// :325-338); a residual above 1e-9 triggers the reprojection (:338-343).
+// Inside the step kernel that check costs a chain-per-CU schedule a fifth of
+// its step: S (0.43 MB at 5,800 x 10,600) is re-read past L2 for every chain
+// and every check, the gathers of the other chains having evicted it, and its
+// three dependent loads per slice wait out memory latency with nothing else
+// to run. Here the step kernel stops every chain at its check with x stored
+// in its row of X, and this kernel evaluates the residuals of all of them:
+// a workgroup stages one chain's x in LDS (one coalesced read of n doubles),
+// its wavefronts sum the rows of S from a sliced ELL table that stays in L2
+// across the chains of the launch, and the next chain's x is loaded into
+// registers while the current one is summed.
+//
+// Each row is summed exactly as the step kernel and the oracle sum it: the
+// row's nonzeros in CSR order, fma(value, x, acc) from +0.0; the chain's
+// residual is the largest absolute row sum (order independent). The flag,
+// the reprojection counter and the FP32 residual rows of the
+// reduced-precision rounds are the ones the in-kernel check writes, so the
+// two placements give identical results.
+#include <algorithm>
+
+#include "achr_internal.h"
+
+namespace achr {
+namespace {
+
+constexpr int kThreads = 1024;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxSlots = kMaxCheckReactions / kThreads;  // x elements per thread
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ rsrc_t row_rsrc(const double* p, int n) {
+  // raw buffer: offsets at or past 8 n bytes read 0.0 without a memory access
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), static_cast<short>(0), 8 * n, 0x00020000);
+}
+
+__device__ __forceinline__ double vmax(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = vmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void residual_check_kernel(CheckArgs a) {
+  extern __shared__ double xs[];  // n doubles of the current chain, then kWaves partial maxima
+  double* red = xs + a.n;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int l = tid & 63;
+  const int w = tid >> 6;
+  const int n = a.n;
+  double pre[kMaxSlots];
+  auto pending = [&](int b) { return b < a.P && a.st[b].z == kPendingCheck; };
+  auto fetch = [&](int b) {
+    const rsrc_t r = row_rsrc(a.X + static_cast<size_t>(b) * n, n);
+#pragma unroll
+    for (int k = 0; k < kMaxSlots; ++k)
+      pre[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (tid + k * kThreads) * 8, 0, 0));
+  };
+  int b = static_cast<int>(blockIdx.x);
+  bool have = pending(b);
+  if (have) fetch(b);
+  while (b < a.P) {
+    const int bn = b + static_cast<int>(gridDim.x);
+    const bool have_n = pending(bn);
+    if (have) {
+#pragma unroll
+      for (int k = 0; k < kMaxSlots; ++k)
+        if (tid + k * kThreads < n) xs[tid + k * kThreads] = pre[k];
+      __syncthreads();
+      if (have_n) fetch(bn);  // in flight while this chain's rows are summed
+      float* rrow = a.R32 ? a.R32 + static_cast<size_t>(b) * a.r32_ld : nullptr;
+      double rho = 0.0;
+      for (int sl = w; sl < a.nslices; sl += kWaves) {
+        const int e0 = __builtin_amdgcn_readfirstlane(a.ell_ptr[sl]);
+        const int width = (__builtin_amdgcn_readfirstlane(a.ell_ptr[sl + 1]) - e0) >> 6;
+        double acc = 0.0;
+        for (int e = 0; e < width; e += 4) {
+          int ci[4];
+          double vv[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int at = e0 + (min(e + q, width - 1) << 6) + l;
+            ci[q] = a.ell_col[at];
+            vv[q] = a.ell_val[at];
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double xv = xs[ci[q] < 0 ? 0 : ci[q]];
+            if (e + q < width && ci[q] >= 0) acc = fma(vv[q], xv, acc);
+          }
+        }
+        const int row = a.ell_row[(sl << 6) + l];
+        if (row >= 0) {
+          const double av = fabs(acc);
+          if (av > rho) rho = av;
+          if (rrow) rrow[row] = static_cast<float>(acc);
+        }
+      }
+      rho = wave_max(rho);
+      if (l == 0) red[w] = rho;
+      __syncthreads();
+      if (tid == 0) {
+        double r = red[0];
+#pragma unroll
+        for (int i = 1; i < kWaves; ++i) r = vmax(r, red[i]);
+        const bool reproj = r > a.reprojTol && a.d > 0;
+        a.flag[b] = reproj ? 1 : 0;
+        if (reproj) ++a.stats[b].y;
+      }
+      // the partials and xs are rewritten only after every wavefront is done
+      // with this chain: the next chain's LDS stores follow this barrier
+      __syncthreads();
+    } else if (have_n) {
+      fetch(bn);
+    }
+    b = bn;
+    have = have_n;
+  }
+}
+
+}  // namespace
+
+size_t check_lds_bytes(int n) { return (static_cast<size_t>(n) + kWaves) * sizeof(double); }
+
+hipError_t launch_check(const CheckArgs& a, hipStream_t stream) {
+  if (a.P <= 0) return hipSuccess;
+  if (a.n <= 0 || a.n > kMaxCheckReactions) return hipErrorInvalidValue;
+  const size_t lds = check_lds_bytes(a.n);
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&residual_check_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  // one workgroup per CU (the staged x fills most of the LDS), each taking
+  // every grid-th chain
+  const int grid = std::min(a.P, std::max(cus, 1));
+  CheckArgs arg = a;
+  void* args[] = {&arg};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&residual_check_kernel), dim3(static_cast<unsigned>(grid)),
+                         dim3(kThreads), args, lds, stream);
+}
+
+}  // namespace achr

@@ -111,7 +111,7 @@ struct achr_sampler {
   // timing
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, int>> ev_step, ev_proj;  // indices into ev_pool
+  std::vector<std::pair<int, int>> ev_step, ev_proj, ev_check;  // indices into ev_pool
   bool have_run = false;
 };
 
@@ -134,6 +134,10 @@ void fill_args(const achr_sampler* S, int P, int Ssteps, uint64_t seed, uint32_t
 // (step launch, then the projection round of checks already done + segment
 // index), recording their events.
 int enqueue_segments(achr_sampler* S, const achr::StepArgs& a, int segments, int checks_done, hipStream_t st);
+// Whether batched rounds of this schedule leave the residual check to
+// launch_check (StepArgs::check_out), and that kernel's arguments for a run.
+bool use_check_kernel(const achr_sampler* S, const achr::Schedule& sc);
+achr::CheckArgs check_args(const achr_sampler* S, const achr::StepArgs& a);
 // Projection round after segment seg (kProjNone outside mode 2).
 int projection_kind(const achr_sampler* S, int seg);
 // Queues one projection round over the chains flagged by the last segment.

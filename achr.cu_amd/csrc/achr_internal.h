@@ -37,6 +37,8 @@ struct StepArgs {
   int mode;              // 1: reprojection inside the kernel; 2: stop at checks (batched GEMM)
   int stop_flagged_only; // mode 2: a chain stops only at a check that flags it, or
   int check_budget;      //   after check_budget checks in one launch (launch balance)
+  int check_out;         // mode 2: every chain stops at its check with x stored; the
+                         //   residual is left to launch_check
   // state, chain major
   double* X;             // P x n: x of chain b at X + b*n
   double* C;             // P x n: running centre
@@ -73,6 +75,29 @@ hipError_t launch_step(const StepArgs& a, const Schedule& s, hipStream_t stream)
 hipError_t prepare_step_batch(const Schedule& s, bool dir_mode, int* resident);
 hipError_t launch_step_batch(const StepArgs* tab, const int2* items, int nitems, int grid, const Schedule& s,
                              bool dir_mode, hipStream_t stream);
+
+// Residual checks of the chains a segment stopped at their check (check_out):
+// for every chain b in phase kPendingCheck, max_j |(S x_b)_j| from the sliced
+// ELL table of S over x_b = X + b*n, the flag (residual > reprojTol and d > 0),
+// the reprojection counter stats[b].y and, if R32 is set, the FP32 row sums.
+constexpr int kMaxCheckReactions = 16384;
+struct CheckArgs {
+  int n, d, P;
+  int nslices;
+  const int* ell_ptr;
+  const int* ell_col;
+  const double* ell_val;
+  const int* ell_row;
+  const double* X;
+  const int4* st;
+  int4* stats;
+  int* flag;
+  float* R32;
+  long r32_ld;
+  double reprojTol;
+};
+size_t check_lds_bytes(int n);
+hipError_t launch_check(const CheckArgs& a, hipStream_t stream);
 
 // Batched reprojection: X[idx[q]] <- N (N^T X[idx[q]]) for q < *count.
 hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStream_t stream);

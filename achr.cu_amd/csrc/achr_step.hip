@@ -613,6 +613,14 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         ++sts.z;
 #pragma unroll
         for (int k = 0; k < E; ++k) xrow.st(lane + k * TPC, x[k]);
+        // the residual of this check is left to the check kernel, which
+        // runs between this launch and the projection round (plain chains)
+        if constexpr (!DIR) {
+          if (a.check_out) {
+            stopped_at_check = true;
+            break;
+          }
+        }
         group_sync<TPC>();
         double rho = 0.0;
         float* rrow = (a.mode == 2 && a.R32) ? a.R32 + static_cast<size_t>(b) * a.r32_ld : nullptr;

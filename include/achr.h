@@ -144,6 +144,11 @@ typedef struct achr_options {
                            round uses ACHR_PROJ_FP64 (10 = every 100 steps) */
   int model_id;         /* Philox counter word 3 is model_id << 4 | tag: models
                            sampled together take distinct ids (0 .. 2^27 - 1) */
+  int check_kernel;     /* where the residual check of batched rounds runs
+                           (reproject_mode 2): 0 = auto (its own kernel for
+                           chains of 512 or more lanes), 1 = inside the step
+                           kernel, 2 = its own kernel between segments (every
+                           chain stops at its check; same results either way) */
 } achr_options;
 
 #define ACHR_PROJ_FP64 0
@@ -250,6 +255,12 @@ int achr_sample_trace(achr_sampler *sampler, double *trace);
  * Also the number of launches of each. */
 int achr_sample_timing(achr_sampler *sampler, double *ms_total, double *ms_step, double *ms_proj,
                        int *n_step_launches, int *n_proj_launches);
+
+/* Milliseconds and launches of the last run's residual-check kernels
+ * (check_kernel 2; zero when the checks ran inside the step kernel). No
+ * counterpart in the reference, whose check is a child launch per chain
+ * (findMaxAbs, ACHRCuda.cu:232-240, :325-338). */
+int achr_sample_timing_check(achr_sampler *sampler, double *ms_check, int *n_check_launches);
 
 /* Schedule the autotuner picked for this sampler: threads per chain,
  * reprojection mode (1 in-kernel, 2 batched). */

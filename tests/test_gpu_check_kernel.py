@@ -1,0 +1,112 @@
+"""The residual check of batched rounds in its own kernel (check_kernel 2).
+
+Every chain stops at its check with x stored; one launch sums the rows of S
+for all of them (x staged in LDS, S from L2) and sets the flags, the
+reprojection counters and, for the reduced-precision rounds, the FP32
+residual rows. The rows are summed in CSR order from +0.0 as the in-kernel
+check and the oracle sum them (findMaxAbs, ACHRcu/ACHRCuda.cu:232-240,
+:325-338), so a run gives the same bits with the check in either place:
+points, counters, traces, and the oracle's."""
+import numpy as np
+import pytest
+
+import achr
+import oracle as O
+from achr import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def genome_case(mets, rxns, seed, exact=False):
+    def ns(rows, cols, vals, m, n):
+        mdl = achr.Model.from_coo(n, m, rows, cols, vals, -np.ones(n), np.ones(n))
+        mdl.compute_nullspace()
+        return mdl.nullspace()
+
+    m = synth.make_model(mets, rxns, seed, irrev_frac=0.4, n_fixed=rxns // 100, width=(100.0, 1000.0),
+                         x0_scale=100.0, nullspace_fn=ns)
+    W = synth.chord_warmup(m.N, m.x0, m.lb, m.ub, 300, seed=seed + 1)
+    if not exact:
+        W = synth.round_like_csv(W)
+    model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
+    model.set_nullspace(m.N)
+    import scipy.sparse as sp
+    Sd = sp.csr_matrix((m.vals, (m.rows, m.cols)), shape=(m.m, m.n))
+    Sd.sort_indices()
+    return m, W, model, Sd
+
+
+def run(model, W, P, S, seed, **opts):
+    smp = achr.Sampler(model, W, achr.Options(reproject_mode=2, **opts))
+    smp.run(P, S, seed=seed)
+    out = (smp.points(), smp.stats(), smp.trace() if opts.get("trace_chains") else None, smp.timing())
+    del smp
+    return out
+
+
+@pytest.mark.parametrize("lanes", [32, 256, 1024])
+def test_check_kernel_on_c1_bitwise(c1, lanes):
+    """Every check of the "%f" warmup reprojects; ragged chain counts."""
+    S_, lb, ub, Wf, _ = c1
+    N = O.nullspace(S_)
+    model = achr.Model.from_dense(S_, lb, ub)
+    model.set_nullspace(N)
+    rp, ci, va = O.csr_of(S_)
+    for P, S in [(1, 31), (77, 123)]:
+        say(f"c1 lanes {lanes} P {P} S {S}")
+        g, gs, _, tm = run(model, Wf, P, S, 11, lanes=lanes, check_kernel=2)
+        o, os_, _ = O.sample(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S_.shape[0], P=P, S=S,
+                             seed=11, lanes=lanes)
+        np.testing.assert_array_equal(gs, os_)
+        np.testing.assert_array_equal(g, o)
+        assert (gs[:, 1] == gs[:, 2]).all()
+        assert tm["check_launches"] == tm["step_launches"] - 1 > 0
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_check_kernel_genome_schedule_equals_in_kernel_and_oracle(exact):
+    """1,024-lane chains at 4,400 x 8,000: with the "%f" warmup every check
+    flags; with exact warmup points only the checks after clipped steps do,
+    so most chains stop at a check without a projection and resume."""
+    say(f"genome case 4400 x 8000, exact warmup {exact}")
+    m, W, model, Sd = genome_case(4400, 8000, 41, exact=exact)
+    P, S, seed = 12, 47, 6
+    a = run(model, W, P, S, seed, lanes=1024, check_kernel=2, trace_chains=3, trace_thin=9)
+    b = run(model, W, P, S, seed, lanes=1024, check_kernel=1, trace_chains=3, trace_thin=9)
+    say(f"timing with the check kernel {a[3]}, inside the step kernel {b[3]}")
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[2], b[2])
+    assert a[3]["check_launches"] > 0 and b[3]["check_launches"] == 0
+    o, ost, otr = O.sample(warm=W, lb=m.lb, ub=m.ub, rowptr=Sd.indptr.astype(np.int32),
+                           colidx=Sd.indices.astype(np.int32), vals=Sd.data, N=m.N, m=m.m, P=P, S=S, seed=seed,
+                           lanes=1024, threads=8, trace_chains=3, trace_thin=9)
+    np.testing.assert_array_equal(a[1], ost)
+    np.testing.assert_array_equal(a[0], o)
+    np.testing.assert_array_equal(a[2], otr)
+    flagged = int(a[1][:, 1].sum())
+    checks = int(a[1][:, 2].sum())
+    say(f"reprojections {flagged} of {checks} checks")
+    if exact:
+        assert flagged < checks
+    else:
+        assert flagged == checks
+
+
+@pytest.mark.parametrize("prec", [achr.PROJ_RES_F32, achr.PROJ_RES_BF16X3])
+def test_check_kernel_residual_rows_of_reduced_precision_rounds(prec):
+    """The residual-form rounds contract the check's FP32 row sums with S+:
+    the check kernel writes the same rows, so the runs are identical."""
+    say(f"reduced precision {prec}, 2200 x 4000, 512 lanes")
+    m, W, model, Sd = genome_case(2200, 4000, 43)
+    P, S, seed = 40, 61, 8
+    a = run(model, W, P, S, seed, lanes=512, check_kernel=2, proj_prec=prec, fp64_every=3)
+    b = run(model, W, P, S, seed, lanes=512, check_kernel=1, proj_prec=prec, fp64_every=3)
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[0], b[0])
+    assert a[3]["check_launches"] > 0
+    assert np.abs(Sd @ a[0]).max() < 1e-3
