@@ -261,7 +261,7 @@ static int sampler_create(const achr_model* M, const double* warmup, int W, cons
   achr_options_default(&o);
   if (opts) o = *opts;
   if (o.check_every <= 0 || o.max_misses < 0) return fail(ACHR_E_ARG, "bad check_every / max_misses");
-  if (o.check_kernel < 0 || o.check_kernel > 2) return fail(ACHR_E_ARG, "bad check_kernel=%d", o.check_kernel);
+  if (o.check_kernel < 0 || o.check_kernel > 3) return fail(ACHR_E_ARG, "bad check_kernel=%d", o.check_kernel);
   if (o.lanes != 0 && !achr::schedule_supported(o.lanes, achr::elems_for(M->n, o.lanes)))
     return fail(ACHR_E_ARG, "lanes=%d cannot hold n=%d reactions", o.lanes, M->n);
   if (o.trace_chains > 0 && o.trace_thin <= 0) return fail(ACHR_E_ARG, "trace_thin must be > 0");
@@ -336,6 +336,23 @@ static int sampler_create(const achr_model* M, const double* warmup, int W, cons
   if (e == hipSuccess) e = S->ell_row.upload(erow.data(), erow.size(), s0);
   if (e == hipSuccess) e = S->ell_col.upload(ecol.data(), ecol.size(), s0);
   if (e == hipSuccess) e = S->ell_val.upload(eval.data(), eval.size(), s0);
+  {
+    achr_host::CheckPlan plan;
+    achr_host::build_check_plan(n, nslices, eptr, erow, ecol, eval, &plan);
+    S->chk_ok = plan.ok;
+    if (plan.ok) {
+      if (e == hipSuccess) e = S->chk_ent.upload(plan.ent.data(), plan.ent.size(), s0);
+      if (e == hipSuccess) e = S->chk_row.upload(plan.rowmap.data(), plan.rowmap.size(), s0);
+      if (e == hipSuccess) e = S->chk_vtab.upload(plan.vtab.data(), plan.vtab.size(), s0);
+      S->chk_nv = static_cast<int>(plan.vtab.size());
+      for (int w = 0; w < achr::kCheckWaves; ++w) {
+        S->chk_wlen[w] = plan.wlen[w];
+        S->chk_woff[w] = plan.woff[w];
+        S->chk_roff[w] = plan.roff[w];
+        S->chk_wends[w] = plan.wends[w];
+      }
+    }
+  }
   if (e == hipSuccess) e = S->warm.upload(warmup, static_cast<size_t>(n) * W, s0);
   if (e == hipSuccess) e = S->lb.upload(M->lb.data(), n, s0);
   if (e == hipSuccess) e = S->ub.upload(M->ub.data(), n, s0);
@@ -657,7 +674,7 @@ bool use_check_kernel(const achr_sampler* S, const achr::Schedule& sc) {
   // the in-kernel check
   if (S->mode != 2 || S->d <= 0 || S->use_dirs || S->n > achr::kMaxCheckReactions || S->opt.check_kernel == 1)
     return false;
-  return S->opt.check_kernel == 2 || sc.tpc >= 512;
+  return S->opt.check_kernel >= 2 || sc.tpc >= 512;
 }
 
 achr::CheckArgs check_args(const achr_sampler* S, const achr::StepArgs& a) {
@@ -677,7 +694,18 @@ achr::CheckArgs check_args(const achr_sampler* S, const achr::StepArgs& a) {
   c.R32 = a.R32;
   c.r32_ld = a.r32_ld;
   c.reprojTol = a.reprojTol;
-  (void)S;
+  if (S->chk_ok && S->opt.check_kernel != 3) {
+    c.ent = S->chk_ent.p;
+    c.rowmap = S->chk_row.p;
+    c.vtab = S->chk_vtab.p;
+    c.nv = S->chk_nv;
+    for (int w = 0; w < achr::kCheckWaves; ++w) {
+      c.wlen[w] = S->chk_wlen[w];
+      c.woff[w] = S->chk_woff[w];
+      c.roff[w] = S->chk_roff[w];
+      c.wends[w] = S->chk_wends[w];
+    }
+  }
   return c;
 }
 

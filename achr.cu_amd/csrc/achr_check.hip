@@ -21,7 +21,10 @@
 // reduced-precision rounds are the ones the in-kernel check writes, so the
 // two placements give identical results.
 #include <algorithm>
+#include <map>
+#include <vector>
 
+#include "achr_host.h"
 #include "achr_internal.h"
 
 namespace achr {
@@ -127,17 +130,121 @@ __global__ __launch_bounds__(kThreads) void residual_check_kernel(CheckArgs a) {
   }
 }
 
+// The resident form: a wavefront's entries of S stay in registers across the
+// chains of the launch, so per chain the only memory reads are the chain's x
+// (staged in LDS as above) and L2 is not involved. Each lane walks its rows of
+// the wavefront's slices one after another; the slice ends are uniform over
+// the wavefront (ELL), marked in a bitmask, where the row's sum is finished.
+template <int R, int XS>
+__global__ __launch_bounds__(kThreads) void residual_check_resident_kernel(CheckArgs a) {
+  extern __shared__ double xs[];  // n doubles of x, nv values of S, kWaves partial maxima
+  double* vt = xs + a.n;
+  double* red = vt + a.nv;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int l = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int n = a.n;
+  for (int i = tid; i < a.nv; i += kThreads) vt[i] = a.vtab[i];
+  const int J = a.wlen[w];
+  const unsigned long long ends = a.wends[w];
+  unsigned ent[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) ent[j] = j < J ? a.ent[a.woff[w] + 64 * j + l] : kCheckPad;
+  const int* rows = a.rowmap + a.roff[w] + l;
+  double pre[XS];  // x of the next chain, XS * kThreads >= n
+  auto pending = [&](int b) { return b < a.P && a.st[b].z == kPendingCheck; };
+  auto fetch = [&](int b) {
+    const rsrc_t r = row_rsrc(a.X + static_cast<size_t>(b) * n, n);
+#pragma unroll
+    for (int k = 0; k < XS; ++k)
+      pre[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (tid + k * kThreads) * 8, 0, 0));
+  };
+  int b = static_cast<int>(blockIdx.x);
+  bool have = pending(b);
+  if (have) fetch(b);
+  while (b < a.P) {
+    const int bn = b + static_cast<int>(gridDim.x);
+    const bool have_n = pending(bn);
+    if (have) {
+#pragma unroll
+      for (int k = 0; k < XS; ++k)
+        if (tid + k * kThreads < n) xs[tid + k * kThreads] = pre[k];
+      __syncthreads();  // also orders the value table before its first use
+      if (have_n) fetch(bn);  // in flight while this chain's rows are summed
+      float* rrow = a.R32 ? a.R32 + static_cast<size_t>(b) * a.r32_ld : nullptr;
+      double rho = 0.0, acc = 0.0;
+      int k = 0;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        if (j < J) {
+          unsigned wd = ent[j];
+          // opaque to the compiler: otherwise it hoists both LDS addresses
+          // and the padding test of every entry out of the chain loop, which
+          // triples the registers the entries occupy
+          asm volatile("" : "+v"(wd));
+          const double xv = xs[wd & 0xFFFFu];
+          const double vv = vt[(wd >> 16) & 0x7FFFu];
+          const double nx = fma(vv, xv, acc);
+          acc = (wd & kCheckPad) ? acc : nx;
+          if ((ends >> j) & 1ull) {
+            const double av = fabs(acc);
+            if (av > rho) rho = av;
+            if (rrow) {
+              const int row = rows[64 * k];
+              if (row >= 0) rrow[row] = static_cast<float>(acc);
+            }
+            acc = 0.0;
+            ++k;
+          }
+        }
+        // four entries at a time: their LDS reads overlap without every
+        // entry's operands being hoisted into registers at once
+        if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      }
+      rho = wave_max(rho);
+      if (l == 0) red[w] = rho;
+      __syncthreads();
+      if (tid == 0) {
+        double r = red[0];
+#pragma unroll
+        for (int i = 1; i < kWaves; ++i) r = vmax(r, red[i]);
+        const bool reproj = r > a.reprojTol && a.d > 0;
+        a.flag[b] = reproj ? 1 : 0;
+        if (reproj) ++a.stats[b].y;
+      }
+      __syncthreads();
+    } else if (have_n) {
+      fetch(bn);
+    }
+    b = bn;
+    have = have_n;
+  }
+}
+
 }  // namespace
 
 size_t check_lds_bytes(int n) { return (static_cast<size_t>(n) + kWaves) * sizeof(double); }
 
+size_t resident_lds_bytes(int n, int nv) { return (static_cast<size_t>(n) + nv + kWaves) * sizeof(double); }
+
 hipError_t launch_check(const CheckArgs& a, hipStream_t stream) {
   if (a.P <= 0) return hipSuccess;
   if (a.n <= 0 || a.n > kMaxCheckReactions) return hipErrorInvalidValue;
-  const size_t lds = check_lds_bytes(a.n);
+  const int xs = (a.n + kThreads - 1) / kThreads;
+  const void* fn = reinterpret_cast<const void*>(&residual_check_kernel);
+  if (a.ent) {
+    if (xs <= 4)
+      fn = reinterpret_cast<const void*>(&residual_check_resident_kernel<kCheckResident, 4>);
+    else if (xs <= 8)
+      fn = reinterpret_cast<const void*>(&residual_check_resident_kernel<kCheckResident, 8>);
+    else if (xs <= 12)
+      fn = reinterpret_cast<const void*>(&residual_check_resident_kernel<kCheckResident, 12>);
+    else
+      fn = reinterpret_cast<const void*>(&residual_check_resident_kernel<kCheckResident, 16>);
+  }
+  const size_t lds = a.ent ? resident_lds_bytes(a.n, a.nv) : check_lds_bytes(a.n);
   if (lds > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&residual_check_kernel),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
   int dev = 0, cus = 0;
@@ -149,8 +256,71 @@ hipError_t launch_check(const CheckArgs& a, hipStream_t stream) {
   const int grid = std::min(a.P, std::max(cus, 1));
   CheckArgs arg = a;
   void* args[] = {&arg};
-  return hipLaunchKernel(reinterpret_cast<const void*>(&residual_check_kernel), dim3(static_cast<unsigned>(grid)),
-                         dim3(kThreads), args, lds, stream);
+  return hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(kThreads), args, lds, stream);
 }
 
 }  // namespace achr
+
+namespace achr_host {
+
+void build_check_plan(int n, int nslices, const std::vector<int>& eptr, const std::vector<int>& erow,
+                      const std::vector<int>& ecol, const std::vector<double>& eval, CheckPlan* plan) {
+  using achr::kCheckWaves;
+  *plan = CheckPlan{};
+  if (n <= 0 || n > achr::kMaxCheckReactions || nslices <= 0) return;
+  // values of S as a table (bit patterns, so -0.0 and 0.0 stay apart)
+  std::map<unsigned long long, int> index;
+  std::vector<double> vtab;
+  for (size_t i = 0; i < ecol.size(); ++i) {
+    if (ecol[i] < 0) continue;
+    const unsigned long long key = __builtin_bit_cast(unsigned long long, eval[i]);
+    if (index.emplace(key, static_cast<int>(vtab.size())).second) vtab.push_back(eval[i]);
+    if (vtab.size() > 0x7FFF) return;
+  }
+  if (vtab.empty()) vtab.push_back(0.0);  // padding words read entry 0
+  if (achr::resident_lds_bytes(n, static_cast<int>(vtab.size())) > 160 * 1024) return;
+  // slices, widest first (their order in the table), each to the wavefront
+  // with the fewest entries so far
+  std::vector<std::vector<int>> mine(kCheckWaves);
+  int load[kCheckWaves] = {};
+  for (int sl = 0; sl < nslices; ++sl) {
+    const int width = (eptr[sl + 1] - eptr[sl]) / 64;
+    if (width == 0) continue;  // rows with no entries: residual 0, nothing to sum
+    int best = 0;
+    for (int w = 1; w < kCheckWaves; ++w)
+      if (load[w] < load[best]) best = w;
+    mine[best].push_back(sl);
+    load[best] += width;
+  }
+  for (int w = 0; w < kCheckWaves; ++w)
+    if (load[w] > achr::kCheckResident) return;
+  for (int w = 0; w < kCheckWaves; ++w) {
+    plan->woff[w] = static_cast<int>(plan->ent.size());
+    plan->roff[w] = static_cast<int>(plan->rowmap.size());
+    plan->wlen[w] = load[w];
+    int j = 0;
+    for (int sl : mine[w]) {
+      const int width = (eptr[sl + 1] - eptr[sl]) / 64;
+      for (int e = 0; e < width; ++e, ++j) {
+        for (int l = 0; l < 64; ++l) {
+          const size_t at = static_cast<size_t>(eptr[sl]) + 64 * static_cast<size_t>(e) + l;
+          const int col = ecol[at];
+          if (col < 0) {
+            plan->ent.push_back(achr::kCheckPad);
+          } else {
+            const int vi = index[__builtin_bit_cast(unsigned long long, eval[at])];
+            plan->ent.push_back(static_cast<unsigned>(col) | (static_cast<unsigned>(vi) << 16));
+          }
+        }
+        if (e == width - 1) plan->wends[w] |= 1ull << j;
+      }
+      for (int l = 0; l < 64; ++l) plan->rowmap.push_back(erow[static_cast<size_t>(sl) * 64 + l]);
+    }
+  }
+  if (plan->ent.empty()) plan->ent.push_back(achr::kCheckPad);
+  if (plan->rowmap.empty()) plan->rowmap.push_back(-1);
+  plan->vtab = std::move(vtab);
+  plan->ok = true;
+}
+
+}  // namespace achr_host

@@ -85,6 +85,14 @@ struct achr_sampler {
   achr_host::DevBuf<int> ell_ptr, ell_col, ell_row;  // S as a sliced ELL table (StepArgs)
   achr_host::DevBuf<double> ell_val;
   int nslices = 0;
+  // resident check plan (achr_host::CheckPlan, device copies)
+  bool chk_ok = false;
+  achr_host::DevBuf<unsigned> chk_ent;
+  achr_host::DevBuf<int> chk_row;
+  achr_host::DevBuf<double> chk_vtab;
+  int chk_nv = 0;
+  int chk_wlen[achr::kCheckWaves] = {}, chk_woff[achr::kCheckWaves] = {}, chk_roff[achr::kCheckWaves] = {};
+  unsigned long long chk_wends[achr::kCheckWaves] = {};
   // reduced-precision projection operands (FP32 copies)
   achr_host::DevBuf<unsigned char> Bpk;  // pseudo-inverse S+ packed for the residual rounds (launch_pack_b)
   int r32_ld = 0;                        // row stride of R32: packed_k(m), zero beyond m
@@ -116,6 +124,19 @@ struct achr_sampler {
 };
 
 namespace achr_host {
+
+// Resident form of the check kernel for a sliced ELL table of S (CheckArgs).
+struct CheckPlan {
+  bool ok = false;  // the table fits: entries per wavefront, value count, LDS
+  std::vector<unsigned> ent;
+  std::vector<int> rowmap;
+  std::vector<double> vtab;
+  int wlen[achr::kCheckWaves] = {}, woff[achr::kCheckWaves] = {}, roff[achr::kCheckWaves] = {};
+  unsigned long long wends[achr::kCheckWaves] = {};
+};
+// eptr/erow/ecol/eval: the sliced ELL table (StepArgs); n reactions.
+void build_check_plan(int n, int nslices, const std::vector<int>& eptr, const std::vector<int>& erow,
+                      const std::vector<int>& ecol, const std::vector<double>& eval, CheckPlan* plan);
 
 // Run set-up and projection rounds shared by single-model runs and batches.
 enum { kProjNone = 0, kProjFP64 = 1, kProjLow = 2 };

@@ -81,6 +81,13 @@ hipError_t launch_step_batch(const StepArgs* tab, const int2* items, int nitems,
 // ELL table of S over x_b = X + b*n, the flag (residual > reprojTol and d > 0),
 // the reprojection counter stats[b].y and, if R32 is set, the FP32 row sums.
 constexpr int kMaxCheckReactions = 16384;
+// The resident form: the slices of S dealt to the kernel's 16 wavefronts
+// (longest first, each to the least loaded wavefront), each wavefront's
+// entries held in registers for the whole launch as packed words
+// col | value index << 16 (bit 31: padding), the values in an LDS table.
+constexpr int kCheckWaves = 16;
+constexpr int kCheckResident = 40;  // entries per lane a wavefront can hold
+constexpr unsigned kCheckPad = 0x80000000u;
 struct CheckArgs {
   int n, d, P;
   int nslices;
@@ -95,8 +102,19 @@ struct CheckArgs {
   float* R32;
   long r32_ld;
   double reprojTol;
+  // resident form (ent != nullptr): wavefront w holds wlen[w] entries per
+  // lane, entry j of lane l at ent[woff[w] + 64 j + l]; bit j of wends[w]
+  // marks the last entry of a slice; the S row of lane l in the k-th slice
+  // of wavefront w is rowmap[roff[w] + 64 k + l] (-1: none)
+  const unsigned* ent;
+  const int* rowmap;
+  const double* vtab;
+  int nv;
+  int wlen[kCheckWaves], woff[kCheckWaves], roff[kCheckWaves];
+  unsigned long long wends[kCheckWaves];
 };
 size_t check_lds_bytes(int n);
+size_t resident_lds_bytes(int n, int nv);  // LDS of the resident form
 hipError_t launch_check(const CheckArgs& a, hipStream_t stream);
 
 // Batched reprojection: X[idx[q]] <- N (N^T X[idx[q]]) for q < *count.

@@ -144,11 +144,14 @@ typedef struct achr_options {
                            round uses ACHR_PROJ_FP64 (10 = every 100 steps) */
   int model_id;         /* Philox counter word 3 is model_id << 4 | tag: models
                            sampled together take distinct ids (0 .. 2^27 - 1) */
-  int check_kernel;     /* where the residual check of batched rounds runs
-                           (reproject_mode 2): 0 = auto (its own kernel for
-                           chains of 512 or more lanes), 1 = inside the step
-                           kernel, 2 = its own kernel between segments (every
-                           chain stops at its check; same results either way) */
+  int check_kernel;     /* where the residual check of batched rounds of plain
+                           chains runs (reproject_mode 2): 0 = auto (its own
+                           kernel for chains of 512 or more lanes), 1 = inside
+                           the step kernel, 2 = its own kernel between
+                           segments (every chain stops at its check; S held
+                           in registers when it fits, else read from L2),
+                           3 = its own kernel, S always read from L2. The
+                           results are the same in every case. */
 } achr_options;
 
 #define ACHR_PROJ_FP64 0

@@ -48,17 +48,19 @@ def run(model, W, P, S, seed, **opts):
     return out
 
 
-@pytest.mark.parametrize("lanes", [32, 256, 1024])
-def test_check_kernel_on_c1_bitwise(c1, lanes):
-    """Every check of the "%f" warmup reprojects; ragged chain counts."""
+@pytest.mark.parametrize("lanes,form", [(32, 2), (256, 2), (1024, 2), (256, 3)])
+def test_check_kernel_on_c1_bitwise(c1, lanes, form):
+    """Every check of the "%f" warmup reprojects; ragged chain counts. Form
+    2 holds S in registers (the 72 x 95 table fits), form 3 reads it from
+    L2."""
     S_, lb, ub, Wf, _ = c1
     N = O.nullspace(S_)
     model = achr.Model.from_dense(S_, lb, ub)
     model.set_nullspace(N)
     rp, ci, va = O.csr_of(S_)
     for P, S in [(1, 31), (77, 123)]:
-        say(f"c1 lanes {lanes} P {P} S {S}")
-        g, gs, _, tm = run(model, Wf, P, S, 11, lanes=lanes, check_kernel=2)
+        say(f"c1 lanes {lanes} form {form} P {P} S {S}")
+        g, gs, _, tm = run(model, Wf, P, S, 11, lanes=lanes, check_kernel=form)
         o, os_, _ = O.sample(warm=Wf, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=S_.shape[0], P=P, S=S,
                              seed=11, lanes=lanes)
         np.testing.assert_array_equal(gs, os_)
@@ -77,11 +79,13 @@ def test_check_kernel_genome_schedule_equals_in_kernel_and_oracle(exact):
     P, S, seed = 12, 47, 6
     a = run(model, W, P, S, seed, lanes=1024, check_kernel=2, trace_chains=3, trace_thin=9)
     b = run(model, W, P, S, seed, lanes=1024, check_kernel=1, trace_chains=3, trace_thin=9)
-    say(f"timing with the check kernel {a[3]}, inside the step kernel {b[3]}")
-    np.testing.assert_array_equal(a[1], b[1])
-    np.testing.assert_array_equal(a[0], b[0])
-    np.testing.assert_array_equal(a[2], b[2])
-    assert a[3]["check_launches"] > 0 and b[3]["check_launches"] == 0
+    c = run(model, W, P, S, seed, lanes=1024, check_kernel=3, trace_chains=3, trace_thin=9)
+    say(f"timing with the check kernel {a[3]}, inside the step kernel {b[3]}, S from L2 {c[3]}")
+    for other in (b, c):
+        np.testing.assert_array_equal(a[1], other[1])
+        np.testing.assert_array_equal(a[0], other[0])
+        np.testing.assert_array_equal(a[2], other[2])
+    assert a[3]["check_launches"] > 0 and c[3]["check_launches"] > 0 and b[3]["check_launches"] == 0
     o, ost, otr = O.sample(warm=W, lb=m.lb, ub=m.ub, rowptr=Sd.indptr.astype(np.int32),
                            colidx=Sd.indices.astype(np.int32), vals=Sd.data, N=m.N, m=m.m, P=P, S=S, seed=seed,
                            lanes=1024, threads=8, trace_chains=3, trace_thin=9)
@@ -100,13 +104,41 @@ def test_check_kernel_genome_schedule_equals_in_kernel_and_oracle(exact):
 @pytest.mark.parametrize("prec", [achr.PROJ_RES_F32, achr.PROJ_RES_BF16X3])
 def test_check_kernel_residual_rows_of_reduced_precision_rounds(prec):
     """The residual-form rounds contract the check's FP32 row sums with S+:
-    the check kernel writes the same rows, so the runs are identical."""
+    the check kernel writes the same rows (both forms), so the runs are
+    identical."""
     say(f"reduced precision {prec}, 2200 x 4000, 512 lanes")
     m, W, model, Sd = genome_case(2200, 4000, 43)
     P, S, seed = 40, 61, 8
     a = run(model, W, P, S, seed, lanes=512, check_kernel=2, proj_prec=prec, fp64_every=3)
     b = run(model, W, P, S, seed, lanes=512, check_kernel=1, proj_prec=prec, fp64_every=3)
+    c = run(model, W, P, S, seed, lanes=512, check_kernel=3, proj_prec=prec, fp64_every=3)
+    for other in (b, c):
+        np.testing.assert_array_equal(a[1], other[1])
+        np.testing.assert_array_equal(a[0], other[0])
+    assert a[3]["check_launches"] > 0
+    assert np.abs(Sd @ a[0]).max() < 1e-3
+
+
+def test_check_kernel_on_p_putida_hub_rows():
+    """P. putida (911 x 1,060): hub metabolites give slices wider than a
+    wavefront holds, so the kernel reads S from L2; the run equals the
+    in-kernel check's and the oracle's."""
+    from conftest import golden
+    model = achr.Model.read_mps(golden("P_Putida.mps"))
+    model.compute_nullspace()
+    N = model.nullspace()
+    lb, ub = model.bounds()
+    # a short warmup set from the LP generator: its points need not be
+    # vertices for a comparison of two placements of the same check
+    W, _, _, _ = achr.warmup_points(model, 2 * model.n, max_iters=2000, polish=False)
+    P, S, seed = 24, 35, 2
+    a = run(model, W, P, S, seed, lanes=512, check_kernel=2)
+    b = run(model, W, P, S, seed, lanes=512, check_kernel=1)
     np.testing.assert_array_equal(a[1], b[1])
     np.testing.assert_array_equal(a[0], b[0])
     assert a[3]["check_launches"] > 0
-    assert np.abs(Sd @ a[0]).max() < 1e-3
+    rp, ci, va = model.csr()
+    o, ost, _ = O.sample(warm=W, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=model.m, P=P, S=S, seed=seed,
+                         lanes=512, threads=8)
+    np.testing.assert_array_equal(a[1], ost)
+    np.testing.assert_array_equal(a[0], o)
