@@ -454,8 +454,8 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const double2 bk = bounds(k);
-      if (x[k] > bk.y) x[k] = bk.y;
-      else if (x[k] < bk.x) x[k] = bk.x;
+      const double xk = x[k];
+      x[k] = xk > bk.y ? bk.y : (xk < bk.x ? bk.x : xk);
     }
     clip_due = false;
   };
@@ -732,8 +732,13 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const double2 bk = bounds(k);
-        if (x[k] > bk.y) x[k] = bk.y;
-        else if (x[k] < bk.x) x[k] = bk.x;
+        // as selects: written as branches, the compiler loads the lower
+        // bound inside the branch, so every slot waits out two dependent
+        // memory round trips
+        const double xk = x[k];
+        const bool above = xk > bk.y;
+        const bool below = xk < bk.x;
+        x[k] = above ? bk.y : (below ? bk.x : xk);
         if constexpr (kCL) {
           cl[lane + k * TPC] = fma(alpha, cl[lane + k * TPC], beta * x[k]);
         } else {
