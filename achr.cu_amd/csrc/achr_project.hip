@@ -53,7 +53,16 @@ constexpr int kColStride = BM + 16; // k-major tile of A: 144 doubles per k
 // depends only on its own row of A and column of B), and k past K reads
 // k = K - 1 and is replaced by 0.0, which adds nothing. Operands contiguous
 // in k are read 16 lanes per row, whole 128-byte lines per instruction.
-template <bool AK>
+//
+// A round of few columns (the flagged chains of one check period) gets only
+// a few column tiles, too few workgroups for the CUs: TBN = 32 is the same
+// kernel with 128 x 32 tiles (wavefront w owns rows 32 w + [0, 32) as 2 x 2
+// blocks), four times the workgroups per column. Both widths are enqueued
+// and each runs only when the round's count (on the device) is in its range
+// (COLS_LE: at most kNarrowCols columns; else more), so the choice needs no
+// host synchronisation. Each output is the same k-ordered chain either way.
+constexpr int kNarrowCols = 1024;
+template <bool AK, int TBN = BN, int COLS_LE = 0>
 __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(int M, int K, const double* __restrict__ A,
                                                                long sar, long sak, const double* __restrict__ B,
                                                                long ldb, const int* __restrict__ idxB,
@@ -61,11 +70,16 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(int M, int K, c
                                                                const int* __restrict__ idxC,
                                                                const int* __restrict__ count) {
   const int ncols = *count;
-  const int q0 = blockIdx.y * BN;
+  if (COLS_LE > 0 && ncols > COLS_LE) return;
+  if (COLS_LE < 0 && ncols <= -COLS_LE) return;
+  const int q0 = blockIdx.y * TBN;
   if (q0 >= ncols) return;
   const int r0 = blockIdx.x * BM;
+  constexpr int kPB = (BK * TBN) / kThreads;  // elements of B per thread and slice
+  constexpr int WM = TBN == BN ? 64 : 32, WN = TBN == BN ? 64 : 32;  // rows / columns per wavefront
+  constexpr int NI = WM / 16, NJ = WN / 16;
   __shared__ double As[2][AK ? BM * kRowStride : BK * kColStride];
-  __shared__ double Bs[2][BN * kRowStride];
+  __shared__ double Bs[2][TBN * kRowStride];
   const int t = threadIdx.x;
 
   // k-contiguous operands: lane k = t % 16 of rows / columns t / 16 + 16 e.
@@ -81,19 +95,19 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(int M, int K, c
     pA[0] = A + static_cast<long>(min(r0 + (t & 127), M - 1)) * sar;
     a_kstride = sak;
   }
-  const double* pB[kPerThread];
+  const double* pB[kPB];
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e) {
+  for (int e = 0; e < kPB; ++e) {
     const int q = min(q0 + lr + 16 * e, ncols - 1);
     pB[e] = B + (idxB ? static_cast<long>(idxB[q]) : static_cast<long>(q)) * ldb;
   }
 
-  double pa[kPerThread], pb[kPerThread];
+  double pa[kPerThread], pb[kPB];
   auto load_slice = [&](int k0) {
     const int kb = k0 + lk;
     const long kc = min(kb, K - 1);
 #pragma unroll
-    for (int e = 0; e < kPerThread; ++e) {
+    for (int e = 0; e < kPB; ++e) {
       const double v = pB[e][kc];
       pb[e] = kb < K ? v : 0.0;
     }
@@ -119,18 +133,19 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(int M, int K, c
         As[st][(lr + 16 * e) * kRowStride + lk] = pa[e];
       else
         As[st][((t >> 7) + 2 * e) * kColStride + (t & 127)] = pa[e];
-      Bs[st][(lr + 16 * e) * kRowStride + lk] = pb[e];
     }
+#pragma unroll
+    for (int e = 0; e < kPB; ++e) Bs[st][(lr + 16 * e) * kRowStride + lk] = pb[e];
   };
 
   const int lane = t & 63, w = t >> 6;
-  const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+  const int wr = TBN == BN ? (w >> 1) * 64 : w * 32, wc = TBN == BN ? (w & 1) * 64 : 0;
   const int fr = lane & 15, fk = lane >> 4;
-  f64x4 acc[4][4];
+  f64x4 acc[NI][NJ];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < NI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NJ; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
   if (K > 0) {  // K == 0 (an empty null space): the outputs are zero
     load_slice(0);
@@ -143,23 +158,24 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(int M, int K, c
     if (more) load_slice(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double av[4], bv[4];
+      double av[NI], bv[NJ];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < NI; ++i) {
         if constexpr (AK)
           av[i] = As[st][(wr + 16 * i + fr) * kRowStride + kk + fk];
         else
           av[i] = As[st][(kk + fk) * kColStride + wr + 16 * i + fr];
       }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[st][(wc + 16 * j + fr) * kRowStride + kk + fk];
+      for (int j = 0; j < NJ; ++j) bv[j] = Bs[st][(wc + 16 * j + fr) * kRowStride + kk + fk];
       // raised wave priority while issuing matrix instructions: the other
       // wavefront's loads and LDS traffic fill the gaps (-1.4% per round)
       __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
       __builtin_amdgcn_s_setprio(0);
     }
     if (more) store_slice(st ^ 1);
@@ -167,12 +183,12 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(int M, int K, c
     st ^= 1;
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < NJ; ++j) {
     const int q = q0 + wc + 16 * j + fr;
     if (q >= ncols) continue;
     double* ccol = Cm + (idxC ? static_cast<long>(idxC[q]) : static_cast<long>(q)) * ldc;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = r0 + wr + 16 * i + fk + 4 * e;
@@ -440,18 +456,40 @@ hipError_t launch_project(const double* Ncm, int n, int d, double* X, double* T,
                           const int* count, int P, hipStream_t stream) {
   const unsigned gq = static_cast<unsigned>((P + BN - 1) / BN);
   if (gq == 0 || n <= 0) return hipSuccess;
+  // rounds of at most kNarrowCols flagged chains take 128 x 32 tiles, larger
+  // ones 128 x 128; the kernel of the other width returns at once
+  const bool narrow = P > kNarrowCols / 4;  // with fewer chains the wide tiles fill the device anyway
+  const unsigned gn = static_cast<unsigned>((std::min(P, kNarrowCols) + 31) / 32);
   // T(:, q) = N^T X(:, idx[q]):   A(j, i) = N[i + j*n]  (sar = n, sak = 1); nothing when d == 0
   if (d > 0) {
-    hipLaunchKernelGGL(dgemm_mfma_kernel<true>, dim3((d + BM - 1) / BM, gq), dim3(kThreads), 0, stream, d, n, Ncm,
-                       static_cast<long>(n), 1L, X, static_cast<long>(n), idx, T, static_cast<long>(d),
-                       static_cast<const int*>(nullptr), count);
+    if (narrow) {
+      hipLaunchKernelGGL((dgemm_mfma_kernel<true, 32, kNarrowCols>), dim3((d + BM - 1) / BM, gn), dim3(kThreads), 0,
+                         stream, d, n, Ncm, static_cast<long>(n), 1L, X, static_cast<long>(n), idx, T,
+                         static_cast<long>(d), static_cast<const int*>(nullptr), count);
+      hipLaunchKernelGGL((dgemm_mfma_kernel<true, BN, -kNarrowCols>), dim3((d + BM - 1) / BM, gq), dim3(kThreads), 0,
+                         stream, d, n, Ncm, static_cast<long>(n), 1L, X, static_cast<long>(n), idx, T,
+                         static_cast<long>(d), static_cast<const int*>(nullptr), count);
+    } else {
+      hipLaunchKernelGGL(dgemm_mfma_kernel<true>, dim3((d + BM - 1) / BM, gq), dim3(kThreads), 0, stream, d, n, Ncm,
+                         static_cast<long>(n), 1L, X, static_cast<long>(n), idx, T, static_cast<long>(d),
+                         static_cast<const int*>(nullptr), count);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   // X(:, idx[q]) = N T(:, q):     A(i, j) = N[i + j*n]  (sar = 1, sak = n)
-  hipLaunchKernelGGL(dgemm_mfma_kernel<false>, dim3((n + BM - 1) / BM, gq), dim3(kThreads), 0, stream, n, d, Ncm, 1L,
-                     static_cast<long>(n), T, static_cast<long>(d), static_cast<const int*>(nullptr), X,
-                     static_cast<long>(n), idx, count);
+  if (narrow) {
+    hipLaunchKernelGGL((dgemm_mfma_kernel<false, 32, kNarrowCols>), dim3((n + BM - 1) / BM, gn), dim3(kThreads), 0,
+                       stream, n, d, Ncm, 1L, static_cast<long>(n), T, static_cast<long>(d),
+                       static_cast<const int*>(nullptr), X, static_cast<long>(n), idx, count);
+    hipLaunchKernelGGL((dgemm_mfma_kernel<false, BN, -kNarrowCols>), dim3((n + BM - 1) / BM, gq), dim3(kThreads), 0,
+                       stream, n, d, Ncm, 1L, static_cast<long>(n), T, static_cast<long>(d),
+                       static_cast<const int*>(nullptr), X, static_cast<long>(n), idx, count);
+  } else {
+    hipLaunchKernelGGL(dgemm_mfma_kernel<false>, dim3((n + BM - 1) / BM, gq), dim3(kThreads), 0, stream, n, d, Ncm,
+                       1L, static_cast<long>(n), T, static_cast<long>(d), static_cast<const int*>(nullptr), X,
+                       static_cast<long>(n), idx, count);
+  }
   return hipGetLastError();
 }
 

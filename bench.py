@@ -245,6 +245,18 @@ def roofline_for(args_proj, m, n, d, P, S, q, tm, bank=False):
                           f"{attempts} attempts, {segs} launches"}
 
 
+def check_kernel_rate(n, q, tm):
+    """The residual-check kernel of the batched rounds (plain chains of 512
+    or more lanes): it reads each checked chain's x once (8n B) and sums the
+    rows of S from registers or L2."""
+    if not tm.get("check_launches"):
+        return None
+    checks = int(q["stats"][:, 2].sum())
+    bytes_ = 8.0 * n * checks
+    return {"ms": tm["ms_check"], "launches": tm["check_launches"], "checks": checks,
+            "GBs_algorithmic": bytes_ / (tm["ms_check"] * 1e-3) / 1e9, "per_check": "8n B (the chain's x)"}
+
+
 def gpu_nullspace(rows, cols, vals, m, n):
     import achr
     mdl = achr.Model.from_coo(n, m, rows, cols, vals, -np.ones(n), np.ones(n))
@@ -949,7 +961,8 @@ def main():
                  "min_ess_traced": plain_q["min_ess"], "drift_inf": plain_q["drift"],
                  "max_rhat": plain_q["max_rhat"], "kernel_ms": tm_p, "projection": args.proj,
                  "reprojections": int(plain_q["stats"][:, 1].sum()), "checks": int(plain_q["stats"][:, 2].sum()),
-                 "warmup": "as read (the reference's behaviour: \"%f\" points, every check reprojects)"}
+                 "warmup_set": "as read (the reference's behaviour: \"%f\" points, every check reprojects)",
+                 "check_kernel": check_kernel_rate(n, plain_q, tm_p)}
         log(f"plain: {plain}")
         del smp
     proj_q = None
@@ -968,8 +981,9 @@ def main():
                       "chain_steps_per_s": P * S / wall_q, "min_ess_traced": proj_q["min_ess"],
                       "drift_inf": proj_q["drift"], "max_rhat": proj_q["max_rhat"], "kernel_ms": tm_q,
                       "reprojections": int(proj_q["stats"][:, 1].sum()), "checks": int(proj_q["stats"][:, 2].sum()),
-                      "warmup": f"projected onto S x = 0 with the fixed reactions at their bound ({t_proj:.1f}s, "
-                                f"untimed), FP64 projection rounds for the checks that still flag"}
+                      "warmup_set": f"projected onto S x = 0 with the fixed reactions at their bound ({t_proj:.1f}s, "
+                                f"untimed), FP64 projection rounds for the checks that still flag",
+                      "check_kernel": check_kernel_rate(n, proj_q, tm_q)}
         log(f"plain, projected warmup: {plain_proj}")
         del smp
     if args.sampler == "plain":

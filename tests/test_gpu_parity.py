@@ -116,6 +116,33 @@ def test_medium_model_batched_gemm_bitwise():
         np.testing.assert_array_equal(g, o)
 
 
+@pytest.mark.parametrize("P,exact", [(600, False), (1300, False), (1300, True)])
+def test_projection_round_widths_bitwise(P, exact):
+    """Batched FP64 rounds pick their tile width on the device from the
+    round's count of flagged chains: 128 x 32 tiles up to 1,024 chains,
+    128 x 128 above (runs of more than 256 chains). Every check flags with the
+    "%f" warmup (600 and 1,300 chains per round), few do with exact points;
+    each output keeps the sequential FMA order, so both widths equal the
+    oracle bit for bit."""
+    from achr import synth
+    import scipy.linalg
+    m = synth.make_model(330, 600, 17, irrev_frac=0.3, n_fixed=6, width=(5.0, 50.0))
+    N = scipy.linalg.null_space(m.dense())
+    W = synth.chord_warmup(N, m.x0, m.lb, m.ub, 400, seed=3)
+    if not exact:
+        W = synth.round_like_csv(W)
+    model = achr.Model.from_coo(m.n, m.m, m.rows, m.cols, m.vals, m.lb, m.ub)
+    model.set_nullspace(N)
+    rp, ci, va = O.csr_of(m.dense())
+    st = dict(S=m.dense(), lb=m.lb, ub=m.ub, N=N, rp=rp, ci=ci, va=va, model=model)
+    g, gs, _, o, os_, _ = run_both(st, W, lanes=64, mode=2, P=P, S=31, seed=7, threads=8)
+    np.testing.assert_array_equal(gs, os_)
+    np.testing.assert_array_equal(g, o)
+    flagged = int(gs[:, 1].sum())
+    print(f"P {P} exact {exact}: {flagged} reprojections of {int(gs[:, 2].sum())} checks", flush=True)
+    assert (flagged < gs[:, 2].sum()) if exact else (flagged == gs[:, 2].sum())
+
+
 def test_gpu_nullspace_matches_scipy(c1_setup):
     import scipy.linalg
     m = achr.Model.read_mps(golden("c1.mps"))
