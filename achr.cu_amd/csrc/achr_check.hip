@@ -53,20 +53,29 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(kThreads) void residual_check_kernel(CheckArgs a) {
-  extern __shared__ double xs[];  // n doubles of the current chain, then kWaves partial maxima
+// NT threads per workgroup, XS slots of the next chain's x per thread
+// (XS * NT >= n). Models of at most 4,096 reactions take 256-thread
+// workgroups, several per CU, so the checks of several chains overlap; the
+// entries of a slice are read eight at a time, the next eight requested
+// before the current ones are summed (long rows of hub metabolites would
+// otherwise wait out an L2 round trip every few entries).
+template <int NT, int XS>
+__global__ __launch_bounds__(NT) void residual_check_kernel(CheckArgs a) {
+  constexpr int NW = NT / 64;
+  constexpr int U = 8;
+  extern __shared__ double xs[];  // n doubles of the current chain, then NW partial maxima
   double* red = xs + a.n;
   const int tid = static_cast<int>(threadIdx.x);
   const int l = tid & 63;
   const int w = tid >> 6;
   const int n = a.n;
-  double pre[kMaxSlots];
+  double pre[XS];
   auto pending = [&](int b) { return b < a.P && a.st[b].z == kPendingCheck; };
   auto fetch = [&](int b) {
     const rsrc_t r = row_rsrc(a.X + static_cast<size_t>(b) * n, n);
 #pragma unroll
-    for (int k = 0; k < kMaxSlots; ++k)
-      pre[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (tid + k * kThreads) * 8, 0, 0));
+    for (int k = 0; k < XS; ++k)
+      pre[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (tid + k * NT) * 8, 0, 0));
   };
   int b = static_cast<int>(blockIdx.x);
   bool have = pending(b);
@@ -76,29 +85,40 @@ __global__ __launch_bounds__(kThreads) void residual_check_kernel(CheckArgs a) {
     const bool have_n = pending(bn);
     if (have) {
 #pragma unroll
-      for (int k = 0; k < kMaxSlots; ++k)
-        if (tid + k * kThreads < n) xs[tid + k * kThreads] = pre[k];
+      for (int k = 0; k < XS; ++k)
+        if (tid + k * NT < n) xs[tid + k * NT] = pre[k];
       __syncthreads();
       if (have_n) fetch(bn);  // in flight while this chain's rows are summed
       float* rrow = a.R32 ? a.R32 + static_cast<size_t>(b) * a.r32_ld : nullptr;
       double rho = 0.0;
-      for (int sl = w; sl < a.nslices; sl += kWaves) {
+      for (int sl = w; sl < a.nslices; sl += NW) {
         const int e0 = __builtin_amdgcn_readfirstlane(a.ell_ptr[sl]);
         const int width = (__builtin_amdgcn_readfirstlane(a.ell_ptr[sl + 1]) - e0) >> 6;
         double acc = 0.0;
-        for (int e = 0; e < width; e += 4) {
-          int ci[4];
-          double vv[4];
+        int ci[U];
+        double vv[U];
+        auto ld = [&](int e) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < U; ++q) {
             const int at = e0 + (min(e + q, width - 1) << 6) + l;
             ci[q] = a.ell_col[at];
             vv[q] = a.ell_val[at];
           }
+        };
+        if (width > 0) ld(0);
+        for (int e = 0; e < width; e += U) {
+          int cc[U];
+          double cv[U];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const double xv = xs[ci[q] < 0 ? 0 : ci[q]];
-            if (e + q < width && ci[q] >= 0) acc = fma(vv[q], xv, acc);
+          for (int q = 0; q < U; ++q) {
+            cc[q] = ci[q];
+            cv[q] = vv[q];
+          }
+          if (e + U < width) ld(e + U);
+#pragma unroll
+          for (int q = 0; q < U; ++q) {
+            const double xv = xs[cc[q] < 0 ? 0 : cc[q]];
+            if (e + q < width && cc[q] >= 0) acc = fma(cv[q], xv, acc);
           }
         }
         const int row = a.ell_row[(sl << 6) + l];
@@ -114,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void residual_check_kernel(CheckArgs a) {
       if (tid == 0) {
         double r = red[0];
 #pragma unroll
-        for (int i = 1; i < kWaves; ++i) r = vmax(r, red[i]);
+        for (int i = 1; i < NW; ++i) r = vmax(r, red[i]);
         const bool reproj = r > a.reprojTol && a.d > 0;
         a.flag[b] = reproj ? 1 : 0;
         if (reproj) ++a.stats[b].y;
@@ -223,6 +243,7 @@ __global__ __launch_bounds__(kThreads) void residual_check_resident_kernel(Check
 
 }  // namespace
 
+constexpr int kSmallCheckN = 4096;  // models up to this size: 256-thread workgroups
 size_t check_lds_bytes(int n) { return (static_cast<size_t>(n) + kWaves) * sizeof(double); }
 
 size_t resident_lds_bytes(int n, int nv) { return (static_cast<size_t>(n) + nv + kWaves) * sizeof(double); }
@@ -231,7 +252,9 @@ hipError_t launch_check(const CheckArgs& a, hipStream_t stream) {
   if (a.P <= 0) return hipSuccess;
   if (a.n <= 0 || a.n > kMaxCheckReactions) return hipErrorInvalidValue;
   const int xs = (a.n + kThreads - 1) / kThreads;
-  const void* fn = reinterpret_cast<const void*>(&residual_check_kernel);
+  const bool small = !a.ent && a.n <= kSmallCheckN;
+  const void* fn = reinterpret_cast<const void*>(&residual_check_kernel<kThreads, kMaxSlots>);
+  if (small) fn = reinterpret_cast<const void*>(&residual_check_kernel<256, kSmallCheckN / 256>);
   if (a.ent) {
     if (xs <= 4)
       fn = reinterpret_cast<const void*>(&residual_check_resident_kernel<kCheckResident, 4>);
@@ -247,16 +270,18 @@ hipError_t launch_check(const CheckArgs& a, hipStream_t stream) {
     const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  int dev = 0, cus = 0;
+  int dev = 0, cus = 0, per_cu = 1;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int threads = small ? 256 : kThreads;
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds);
   if (e != hipSuccess) return e;
-  // one workgroup per CU (the staged x fills most of the LDS), each taking
-  // every grid-th chain
-  const int grid = std::min(a.P, std::max(cus, 1));
+  // as many workgroups as stay resident (one per CU when the staged x fills
+  // most of the LDS), each taking every grid-th chain
+  const int grid = std::min(a.P, std::max(cus, 1) * std::max(per_cu, 1));
   CheckArgs arg = a;
   void* args[] = {&arg};
-  return hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(kThreads), args, lds, stream);
+  return hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(threads), args, lds, stream);
 }
 
 }  // namespace achr
