@@ -64,6 +64,7 @@ SIGNATURES = {
     "achr_sample_trace": (_I, [_P, _PD]),
     "achr_sample_timing": (_I, [_P, _PD, _PD, _PD, _PI, _PI]),
     "achr_sample_timing_check": (_I, [_P, _PD, _PI]),
+    "achr_read_bandwidth": (_I, [C.c_long, _I, _PD]),
     "achr_sampler_schedule": (_I, [_P, _PI, _PI]),
     "achr_write_points_csv": (_I, [C.c_char_p, _PD, _I, _I]),
     "achr_write_points_bin": (_I, [C.c_char_p, _PD, _I, _I]),

@@ -1010,6 +1010,35 @@ int achr_sample_timing_check(achr_sampler* S, double* ms_check, int* n_check) {
   return ACHR_OK;
 }
 
+int achr_read_bandwidth(long bytes, int reps, double* gb_per_s) {
+  if (bytes < (1L << 20) || reps < 1 || !gb_per_s) return fail(ACHR_E_ARG, "bad read-rate arguments");
+  const size_t words = static_cast<size_t>(bytes) / 16;  // 16-byte loads
+  DevBuf<double> buf, sums;
+  HIP_TRY(buf.ensure(words * 2));
+  HIP_TRY(hipMemset(buf.p, 0, words * 16));
+  int blocks = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(achr::read_stream_blocks(&blocks));
+  HIP_TRY(sums.ensure(static_cast<size_t>(blocks)));
+  HIP_TRY(hipEventCreate(&e0));
+  hipError_t e = hipEventCreate(&e1);
+  float best = 1e30f;
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = achr::launch_read_stream(buf.p, words, sums.p, blocks, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && ms < best) best = ms;
+  }
+  (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(ACHR_E_DEVICE, "read-rate kernel: %s", hipGetErrorString(e));
+  *gb_per_s = 16.0 * static_cast<double>(words) / (best * 1e-3) / 1e9;
+  return ACHR_OK;
+}
+
 int achr_bank_reciprocal(const double* u, double* r, long count) {
   if (!u || !r || count < 0) return fail(ACHR_E_ARG, "bad reciprocal arguments");
   if (count == 0) return ACHR_OK;

@@ -152,6 +152,11 @@ hipError_t launch_pinv_csc(int n, int m, const int* colptr, const int* rowidx, c
 hipError_t launch_to_f32(const double* src, float* dst, size_t count, hipStream_t stream);
 // q[i] = a[i] / b[i] by the plain chord's reciprocal-and-correction (device arrays).
 hipError_t launch_quotient(const double* a, const double* b, double* q, size_t count, hipStream_t stream);
+// Read-only stream over `words` 16-byte words of `buf` (a grid-stride sum
+// per thread, one double per workgroup into sums[blocks]); blocks = enough
+// resident workgroups to fill the device.
+hipError_t read_stream_blocks(int* blocks);
+hipError_t launch_read_stream(const double* buf, size_t words, double* sums, int blocks, hipStream_t stream);
 // r[i] = the bank-mode chord's reciprocal of u[i] (device arrays).
 hipError_t launch_bank_reciprocal(const double* u, double* r, size_t count, hipStream_t stream);
 

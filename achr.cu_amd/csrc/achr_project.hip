@@ -452,6 +452,43 @@ hipError_t launch_compact(const int* flag, int P, int* idx, int* count, hipStrea
   return hipGetLastError();
 }
 
+namespace {
+// Four 16-byte loads in flight per thread and iteration, consecutive threads
+// on consecutive words: each wavefront reads whole 1 KiB runs.
+__global__ __launch_bounds__(256) void read_stream_kernel(const double2* __restrict__ buf, size_t words,
+                                                         double* __restrict__ sums) {
+  __shared__ double part[4];
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  double acc = 0.0;
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < words; i += 4 * stride) {
+    const double2 a = buf[i], b = buf[i + stride], c = buf[i + 2 * stride], d = buf[i + 3 * stride];
+    acc += (a.x + a.y) + (b.x + b.y) + (c.x + c.y) + (d.x + d.y);
+  }
+  for (; i < words; i += stride) acc += buf[i].x + buf[i].y;
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) sums[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+}  // namespace
+
+hipError_t read_stream_blocks(int* blocks) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&read_stream_kernel), 256, 0);
+  if (e == hipSuccess) *blocks = std::max(cus, 1) * std::max(per_cu, 1);
+  return e;
+}
+
+hipError_t launch_read_stream(const double* buf, size_t words, double* sums, int blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(read_stream_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream,
+                     reinterpret_cast<const double2*>(buf), words, sums);
+  return hipGetLastError();
+}
+
 hipError_t launch_project(const double* Ncm, int n, int d, double* X, double* T, const int* idx,
                           const int* count, int P, hipStream_t stream) {
   const unsigned gq = static_cast<unsigned>((P + BN - 1) / BN);

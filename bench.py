@@ -168,6 +168,16 @@ def measure_copy_gbs(mib=1024, reps=5):
     return 2.0 * (mib << 20) / best / 1e9
 
 
+def measure_read_gbs(gib=4, reps=5):
+    """Read-only streaming rate (one pass over `gib` GiB of device memory,
+    best of `reps`): the yardstick for kernels that mostly read."""
+    import ctypes as C
+    import achr
+    v = C.c_double()
+    achr._lib.check(achr.lib().achr_read_bandwidth(gib << 30, reps, C.byref(v)), "achr_read_bandwidth")
+    return v.value
+
+
 def measure(smp, P, S, steps, warmup, seed0=1):
     """Warmup files (20 steps each) then `steps` timed files; returns (wall, timing) of the slowest."""
     for i in range(warmup):
@@ -1039,6 +1049,7 @@ def main():
                     "kernel_ms": tm})
         out["roofline"] = roofline_for("fp64", m, n, d, P, S, q, tm, bank=True)
         out["roofline"]["measured_copy_GBs"] = measure_copy_gbs()
+        out["roofline"]["measured_read_GBs"] = measure_read_gbs()
         meas = PMC_TRAFFIC.get((n, P, args.check_every))
         if meas and out["roofline"]["kernel"] == "step_kernel":
             out["roofline"]["traffic"] = meas[0]

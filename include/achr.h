@@ -500,6 +500,13 @@ int achr_bank_reciprocal(const double *u, double *r, long count);
  * to that. */
 int achr_quotient(const double *a, const double *b, double *q, long count);
 
+/* Measurement aid, no reference counterpart: the HBM read rate a plain
+ * streaming kernel reaches on this device (GB/s, best of `reps` passes over
+ * `bytes` of device memory; every byte read once per pass, nothing written
+ * but one double per workgroup). The yardstick for kernels that mostly
+ * read, beside a device-to-device copy's rate. */
+int achr_read_bandwidth(long bytes, int reps, double *gb_per_s);
+
 /* The exact finish the warmup generator applies to LPs its first-order
  * iteration leaves unconverged: min c^T x s.t. x = N z, lb <= x <= ub, with N
  * (n x d, column major) an orthonormal basis of null(S), by a primal
