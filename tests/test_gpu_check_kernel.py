@@ -142,3 +142,45 @@ def test_check_kernel_on_p_putida_hub_rows():
                          lanes=512, threads=8)
     np.testing.assert_array_equal(a[1], ost)
     np.testing.assert_array_equal(a[0], o)
+
+
+def test_check_kernel_without_rows_and_with_empty_rows():
+    """A box (no rows of S: the residual is 0 and nothing is projected) and a
+    model whose S has empty rows (slices of width 0, left out of the
+    resident plan) at 1,024 lanes: the check kernel's runs equal the
+    in-kernel check's and the oracle's."""
+    rng = np.random.default_rng(9)
+    n = 2500
+    lb = -rng.uniform(1.0, 5.0, n)
+    ub = rng.uniform(1.0, 5.0, n)
+    W = lb[:, None] + (ub - lb)[:, None] * rng.uniform(0.2, 0.8, (n, 64))
+    box = achr.Model.from_coo(n, 0, [], [], [], lb, ub)
+    box.set_nullspace(np.eye(n))
+    # rows 0..9 empty, then 40 rows coupling random pairs of reactions
+    m = 50
+    rows = np.repeat(np.arange(10, m), 2)
+    cols = rng.choice(n, size=2 * (m - 10), replace=False)
+    vals = np.tile([1.0, -1.0], m - 10)
+    Sd = np.zeros((m, n))
+    Sd[rows, cols] = vals
+    N = O.nullspace(Sd)
+    x0 = N @ rng.standard_normal(N.shape[1])
+    x0 *= 0.5 / np.abs(x0).max()
+    Wc = x0[:, None] + N @ (0.1 * rng.standard_normal((N.shape[1], 64)))
+    coupled = achr.Model.from_coo(n, m, rows, cols, vals, lb, ub)
+    coupled.set_nullspace(N)
+    rp, ci, va = O.csr_of(Sd)
+    for name, model, Wm, Nm, mm, (r_, c_, v_) in (
+            ("box", box, W, np.eye(n), 0, O.csr_of(np.zeros((0, n)))),
+            ("empty rows", coupled, Wc, N, m, (rp, ci, va))):
+        say(f"{name}: n {n}, m {mm}")
+        P, S, seed = 9, 33, 4
+        a = run(model, Wm, P, S, seed, lanes=1024, check_kernel=2)
+        b = run(model, Wm, P, S, seed, lanes=1024, check_kernel=1)
+        np.testing.assert_array_equal(a[1], b[1])
+        np.testing.assert_array_equal(a[0], b[0])
+        assert a[3]["check_launches"] > 0
+        o, ost, _ = O.sample(warm=Wm, lb=lb, ub=ub, rowptr=r_, colidx=c_, vals=v_, N=Nm, m=mm, P=P, S=S,
+                             seed=seed, lanes=1024, threads=8)
+        np.testing.assert_array_equal(a[1], ost)
+        np.testing.assert_array_equal(a[0], o)
