@@ -52,13 +52,19 @@ constexpr int kRedSum = 0, kRedMinMax0 = 1, kRedMax = 3, kRedMinMax1 = 4, kRedAr
 // from L2 each step (holding them in registers instead spills). 9 slots of
 // pairs fill 144 KiB at 1,024 lanes, and at fewer lanes as much per thread,
 // so a CU still holds 1,024 such threads. A plain chain with more than 8
-// slots uses that LDS for its centre instead.
+// slots uses that LDS for its centre (8 E bytes per thread) and the bounds
+// of as many first slots as fit beside it in a thread's 159 bytes.
 constexpr int kLdsBoundSlots = 9;
+constexpr int kLdsBytesPerThread = 159;
 
 template <int TPC, int E, int CPB, bool DIR>
 struct BoundPlan {
+  static constexpr bool centre = !DIR && E > 8;
   static constexpr bool resident = CPB == 1 && (DIR || E <= 8);
-  static constexpr int lds = resident ? (E < kLdsBoundSlots ? E : kLdsBoundSlots) : 0;
+  static constexpr int beside = (kLdsBytesPerThread - 8 * E) / 16;
+  static constexpr int lds = resident ? (E < kLdsBoundSlots ? E : kLdsBoundSlots)
+                                      : (CPB == 1 && centre && beside > 0 ? (beside < E ? beside : E) : 0);
+  static constexpr int base = centre ? TPC * E : 0;  // doubles past the reduction arrays
 };
 
 // Buffer-resource loads: 32-bit byte offsets (one VGPR per slot, shared by
@@ -141,11 +147,15 @@ __device__ __forceinline__ double group_sum(double v, double* red) {
   return v;
 }
 
-// Maximum and minimum of two doubles that are never NaN (chord ends,
-// +-infinity, absolute residuals; never a signed zero either): the same bits
-// as compare-and-assign. Written as the instructions themselves: fmax / fmin
-// would first quiet possible signalling NaNs in every operand that comes from
-// a shuffle or from LDS, one more instruction per operand.
+// v_max_f64 / v_min_f64: the IEEE 754-2019 maximum and minimum of two
+// doubles, with -0 ordered below +0 on both operand orders and a NaN operand
+// giving the other one (denormals kept; checked on the device). For values
+// that are never NaN nor signed zeros of opposite sign (chord ends,
+// +-infinity, absolute residuals) that is the same bits as compare-and-
+// assign; the bank-mode clip relies on the signed-zero order itself.
+// Written as the instructions themselves: fmax / fmin would first quiet
+// possible signalling NaNs in every operand that comes from a shuffle or
+// from LDS, one more instruction per operand.
 __device__ __forceinline__ double vmax(double a, double b) {
   double r;
   asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
@@ -360,7 +370,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
   const rsrc_t r_lb = make_rsrc(a.lb, 8LL * n);
   const rsrc_t r_ub = make_rsrc(a.ub, 8LL * n);
   using BP = BoundPlan<TPC, E, CPB, DIR>;
-  double2* bl = reinterpret_cast<double2*>(lds + kRedArrays * kRedSlots);  // (lb, ub) of the LDS slots
+  double2* bl = reinterpret_cast<double2*>(lds + kRedArrays * kRedSlots + BP::base);  // (lb, ub) of the LDS slots
 #pragma unroll
   for (int k = 0; k < BP::lds; ++k) {
     const int off = (lane + k * TPC) * 8;
@@ -444,8 +454,12 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
     }
   };
   load_dir(attempt);
-  // Bank mode clips the last accepted step lazily (at the next bounds loop);
-  // points that leave the kernel are clipped here first. clip_due is
+  // Bank mode clips x to its bounds at the top of every attempt's bounds
+  // loop, with the IEEE 754-2019 maximum and minimum (v_max_f64 /
+  // v_min_f64: -0 orders below +0, a NaN operand yields the other): two
+  // instructions per slot and no condition. The clip is idempotent, so
+  // clipping after a miss or twice changes nothing; points that leave the
+  // kernel after an accepted step are clipped here first. clip_due is
   // uniform over the chain: set by an accepted step (or a resume after the
   // check of one), cleared by a clip.
   bool clip_due = pending;
@@ -454,8 +468,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const double2 bk = bounds(k);
-      const double xk = x[k];
-      x[k] = xk > bk.y ? bk.y : (xk < bk.x ? bk.x : xk);
+      x[k] = vmin(vmax(x[k], bk.x), bk.y);
     }
     clip_due = false;
   };
@@ -534,14 +547,7 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         if (kLook > 0 && k + kLook < E && k + kLook >= BP::lds) bq[k + kLook] = bounds(k + kLook);
         const double2 bk = (kLook > 0 && k >= BP::lds) ? bq[k] : bounds(k);
         const double lbi = bk.x, ubi = bk.y;
-        if constexpr (DIR) {
-          // the clip of the last accepted step, deferred to here where the
-          // bounds are at hand (only after an accepted step: the starting
-          // point is never clipped)
-          const bool above = clip_due && x[k] > ubi;
-          const bool below = clip_due && x[k] < lbi;
-          x[k] = above ? ubi : (below ? lbi : x[k]);
-        }
+        if constexpr (DIR) x[k] = vmin(vmax(x[k], lbi), ubi);  // the attempt's clip (above)
         const double du = ubi - x[k];
         const double dl = x[k] - lbi;
         // Both branches of the reference (ACHRCuda.cu:270-279) divide the same
@@ -928,12 +934,13 @@ hipError_t launch_bank_reciprocal(const double* u, double* r, size_t count, hipS
 
 size_t step_lds_bytes(bool dir, const Schedule& s) {
   size_t bytes = kRedArrays * kRedSlots * sizeof(double);
-  if (s.e > 8 && !dir) bytes += static_cast<size_t>(s.cpb) * s.tpc * s.e * sizeof(double);  // centres in LDS
-  if (s.cpb == 1 && (dir || s.e <= 8)) {
-    // resident (lb, ub) pairs (BoundPlan)
-    bytes += static_cast<size_t>(std::min(s.e, kLdsBoundSlots)) * s.tpc * 2 * sizeof(double);
-  }
-  return bytes;
+  const bool centre = s.e > 8 && !dir;
+  if (centre) bytes += static_cast<size_t>(s.cpb) * s.tpc * s.e * sizeof(double);  // centres in LDS
+  // (lb, ub) pairs of the first slots (BoundPlan)
+  int rows = 0;
+  if (s.cpb == 1 && (dir || s.e <= 8)) rows = std::min(s.e, kLdsBoundSlots);
+  else if (s.cpb == 1 && centre) rows = std::max(0, std::min(s.e, (kLdsBytesPerThread - 8 * s.e) / 16));
+  return bytes + static_cast<size_t>(rows) * s.tpc * 2 * sizeof(double);
 }
 
 hipError_t launch_step(const StepArgs& a, const Schedule& s, hipStream_t stream) {

@@ -42,8 +42,11 @@
  *
  * Bank mode (oracle_sample with a bank): the library's learned sampler —
  * the direction of an attempt is a stored unit column of the bank (picked
- * like a warmup column), with no normalisation and no centre; everything
- * else is the loop above. No counterpart in the reference.
+ * like a warmup column), with no normalisation and no centre, and x is
+ * clipped with the IEEE 754-2019 maximum and minimum (-0 below +0, see
+ * clip_ieee) at the start of every attempt and after every accepted step
+ * (the kernel's v_max_f64 / v_min_f64); everything else is the loop above.
+ * No counterpart in the reference.
  *
  * Deliberate differences from the reference (all documented behaviour of the
  * new sampler, not approximations):
@@ -225,6 +228,21 @@ typedef struct {
   int trace_chains, trace_thin, trace_len;
 } oracle_opts;
 
+/* The IEEE 754-2019 maximum / minimum of two doubles: -0 orders below +0,
+ * and a NaN operand gives the other one. clip_ieee(x, lb, ub) =
+ * min(max(x, lb), ub) is idempotent. */
+static double max_ieee(double a, double b) {
+  if (a > b || isnan(b)) return a;
+  if (b > a || isnan(a)) return b;
+  return signbit(a) ? b : a; /* equal: +0 over -0 */
+}
+static double min_ieee(double a, double b) {
+  if (a < b || isnan(b)) return a;
+  if (b < a || isnan(a)) return b;
+  return signbit(a) ? a : b; /* equal: -0 over +0 */
+}
+static double clip_ieee(double x, double lb, double ub) { return min_ieee(max_ieee(x, lb), ub); }
+
 /* Per-chain counters written to stats[4*b + k]:
  *   k=0 misses, k=1 reprojections, k=2 residual checks, k=3 flags (1 = stuck) */
 static void run_chain(const oracle_model *M, const oracle_opts *o, int P, int S, int b, double *out,
@@ -260,6 +278,7 @@ static void run_chain(const oracle_model *M, const oracle_opts *o, int P, int S,
     for (int i = 0; i < n; ++i) {
       const double ui = M->bank ? u[i] : u[i] / nrm;
       u[i] = ui;
+      if (M->bank) x[i] = clip_ieee(x[i], M->lb[i], M->ub[i]);
       const double du = M->ub[i] - x[i], dl = x[i] - M->lb[i];
       if (du > o->dTol && dl > o->dTol && (ui > o->uTol || ui < -o->uTol)) {
         /* reference (ACHRCuda.cu:270-279): the two distances divided by u_i;
@@ -318,9 +337,13 @@ static void run_chain(const oracle_model *M, const oracle_opts *o, int P, int S,
     const double alpha = (double)(M->W + s) / (double)(M->W + s + 1);
     const double beta = 1.0 / (double)(M->W + s + 1);
     for (int i = 0; i < n; ++i) {
+      if (M->bank) {
+        x[i] = clip_ieee(x[i], M->lb[i], M->ub[i]);
+        continue;
+      }
       if (x[i] > M->ub[i]) x[i] = M->ub[i];
       else if (x[i] < M->lb[i]) x[i] = M->lb[i];
-      if (!M->bank) c[i] = fma(alpha, c[i], beta * x[i]);
+      c[i] = fma(alpha, c[i], beta * x[i]);
     }
     s++;
     if (trace && b < o->trace_chains && o->trace_thin > 0 && s % o->trace_thin == 0) {

@@ -75,6 +75,27 @@ def c1():
     return load_c1()
 
 
+def signed_zero_bank_case(seed=5):
+    """Six reactions, two of them blocked by S (x_0 = x_1 = 0): the blocked
+    coordinates start at +0.0 and no direction moves them. Reaction 0 has
+    the upper bound -0.0, reaction 1 the lower bound +0.0. Returns (S, lb, ub,
+    N, warmup, start point, bank)."""
+    n = 6
+    S = np.zeros((2, n))
+    S[0, 0] = S[1, 1] = 1.0
+    lb = np.array([-1.0, 0.0, -1.0, -2.0, -1.0, -1.0])
+    ub = np.array([-0.0, 1.0, 1.0, 2.0, 1.0, 1.0])
+    N = np.eye(n)[:, 2:].copy()
+    rng = np.random.default_rng(seed)
+    W = np.zeros((n, 12))
+    W[2:] = lb[2:, None] + (ub - lb)[2:, None] * rng.uniform(0.2, 0.8, (n - 2, 12))
+    c0 = W.mean(axis=1)
+    bank = N @ rng.standard_normal((N.shape[1], 20))
+    bank /= np.linalg.norm(bank, axis=0)
+    assert not np.signbit(c0[:2]).any() and not np.signbit(bank[:2]).any()
+    return S, lb, ub, N, W, c0, bank
+
+
 def has_gpu() -> bool:
     try:
         import torch

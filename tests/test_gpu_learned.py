@@ -106,8 +106,38 @@ def test_bank_mode_bitwise_against_oracle(c1):
                                S=150, seed=4, lanes=lanes, bank=bank, trace_chains=5, trace_thin=7,
                                c0=smp.start_point())
         np.testing.assert_array_equal(smp.stats(), ost)
-        np.testing.assert_array_equal(g, o)
-        np.testing.assert_array_equal(smp.trace(), otr)  # recorded points are the clipped ones
+        np.testing.assert_array_equal(g.view(np.uint64), o.view(np.uint64))
+        np.testing.assert_array_equal(smp.trace().view(np.uint64), otr.view(np.uint64))  # the clipped points
+
+
+def test_bank_clip_signed_zeros_bitwise():
+    """The bank clip is v_max_f64 / v_min_f64 (-0 below +0) in the kernel and
+    the same IEEE order in the oracle: a blocked reaction at +0.0 under the
+    upper bound -0.0 ends at -0.0 on both, compared as bit patterns (an
+    equality test of values would not see the sign of a zero), for both
+    bound placements (LDS at 1,024 lanes, memory at 32) and both projection
+    modes."""
+    import oracle as O
+    from conftest import signed_zero_bank_case
+    S, lb, ub, N, W, c0, bank = signed_zero_bank_case()
+    model = achr.Model.from_dense(S, lb, ub)
+    model.set_nullspace(N)
+    rp, ci, va = O.csr_of(S)
+    for lanes, mode in ((32, 1), (1024, 2)):
+        print(f"signed zeros lanes={lanes} mode={mode}", flush=True)
+        smp = achr.Sampler(model, W, achr.Options(lanes=lanes, reproject_mode=mode, trace_chains=3, trace_thin=7))
+        smp.set_directions(bank)
+        smp.use_directions(True)
+        s0 = smp.start_point()
+        assert (s0.view(np.uint64)[:2] == 0).all()
+        g = smp.sample(40, 60, seed=9)
+        o, ost, otr = O.sample(warm=W, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=2, P=40, S=60, seed=9,
+                               lanes=lanes, bank=bank, trace_chains=3, trace_thin=7, c0=s0)
+        np.testing.assert_array_equal(smp.stats(), ost)
+        np.testing.assert_array_equal(g.view(np.uint64), o.view(np.uint64))
+        np.testing.assert_array_equal(smp.trace().view(np.uint64), otr.view(np.uint64))
+        assert (g.view(np.uint64)[0] == np.uint64(0x8000000000000000)).all()
+        assert (g.view(np.uint64)[1] == 0).all()
 
 
 def test_bank_mode_edge_sizes(c1):

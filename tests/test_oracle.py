@@ -136,6 +136,30 @@ def test_samples_feasible_and_clipped(c1):
     assert np.abs(S @ pts2).max() < 1e-9
 
 
+def test_bank_clip_orders_signed_zeros():
+    """Bank mode clips with the IEEE 754-2019 minimumNumber / maximumNumber,
+    which order -0 below +0 (the kernel's v_min_f64 / v_max_f64): a blocked
+    reaction at +0.0 under the upper bound -0.0 ends at -0.0, where a
+    compare-and-select clip (+0 > -0 is false) would keep +0.0; a reaction at
+    +0.0 on the lower bound +0.0 stays +0.0. Every point is inside its
+    bounds, and the moving reactions are unaffected by the zeros."""
+    from conftest import signed_zero_bank_case
+    S, lb, ub, N, W, c0, bank = signed_zero_bank_case()
+    rp, ci, va = O.csr_of(S)
+    pts, st, _ = O.sample(warm=W, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=2, P=8, S=60, seed=9,
+                          lanes=32, bank=bank, c0=c0)
+    assert (st[:, 0] < 60).all() and (st[:, 3] == 0).all()
+    bits = pts.view(np.uint64)
+    assert (bits[0] == np.uint64(0x8000000000000000)).all()
+    assert (bits[1] == 0).all()
+    assert (pts >= lb[:, None]).all() and (pts <= ub[:, None]).all()
+    assert np.abs(pts[2:] - c0[2:, None]).min() > 0  # the free reactions moved
+    # zero steps: the start point as it is (no clip without an attempt)
+    pts0, _, _ = O.sample(warm=W, lb=lb, ub=ub, rowptr=rp, colidx=ci, vals=va, N=N, m=2, P=2, S=0, seed=9,
+                          lanes=32, bank=bank, c0=c0)
+    assert (pts0.view(np.uint64)[0] == 0).all()
+
+
 def test_reduction_lanes_change_only_rounding(c1):
     """Different lane trees reorder one sum; trajectories agree to rounding."""
     S, lb, ub, Wf, We = c1
