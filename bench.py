@@ -46,11 +46,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "achr.cu_amd"))
 
 PROJ = {"fp64": 0, "res_bf16x3": 1, "res_f32": 2, "null_f32": 3, "null_bf16x3": 4}
-# HBM bytes of one full segment launch of the learned step kernel, measured with
+# HBM bytes of one full launch of the learned step kernel (10 check periods: with
+# FP64 rounds an unflagged chain runs through 10 checks per launch), measured with
 # rocprofv3 FETCH_SIZE / WRITE_SIZE and calibrated on a 1 GiB device copy
-# (profiles/step_kernel_bank_genome_pmc_traffic.md), keyed by
-# (reactions, chains, steps per launch); other shapes report null.
-PMC_TRAFFIC = {(10600, 20000, 100): (179.4e9, 173.0e9, "profiles/step_kernel_bank_genome_pmc_traffic.md")}
+# (profiles/step_kernel_bank_genome_pmc_traffic.md), with the algorithmic bytes of
+# that launch, keyed by (reactions, chains, check period); other shapes report null.
+PMC_TRAFFIC = {(10600, 20000, 100): (1761.1e9, 1699.4e9, "profiles/step_kernel_bank_genome_pmc_traffic.md")}
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
@@ -232,7 +233,14 @@ def roofline_for(args_proj, m, n, d, P, S, q, tm, bank=False):
     attempts = P * S + int(q["stats"][:, 0].sum())
     segs = tm["step_launches"]
     state = 16.0 if bank else 32.0
-    step_bytes = 8.0 * n * attempts + state * n * P * max(segs - 1, 0)
+    if bank and args_proj == "fp64" and reproj == 0:
+        # FP64 rounds: a chain no check flags stops only after every 10th check of a launch
+        # (the library's check budget), so it resumed checks // 10 times; the launches
+        # enqueued after the chains finished are empty
+        resumed = int((q["stats"][:, 2] // 10).sum())
+    else:
+        resumed = P * max(segs - 1, 0)  # every chain stops at every check
+    step_bytes = 8.0 * n * attempts + state * n * resumed
     step_ms, proj_ms = tm["ms_step"], tm["ms_proj"]
     if args_proj == "fp64":
         proj_flops = 4.0 * n * d * reproj
@@ -252,7 +260,7 @@ def roofline_for(args_proj, m, n, d, P, S, q, tm, bank=False):
     return {"bound": "hbm", "kernel": "step_kernel", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS, "traffic": None, "launches": segs,
             "per_launch": f"8n B per attempt (direction gather) + {int(state)}n B per chain per resumed segment; "
-                          f"{attempts} attempts, {segs} launches"}
+                          f"{attempts} attempts, {resumed} chain resumptions, {segs} launches"}
 
 
 def check_kernel_rate(n, q, tm):
@@ -1028,8 +1036,18 @@ def main():
             t = time.time()
             rep = smp.learn_directions(rank=args.rank, width=512, iters=args.iters, batch=2048, lr=1e-3, seed=7)
             t_learn = time.time() - t
+            bank = smp.directions()
             if args.save_inputs:
-                np.save(os.path.join(args.save_inputs, "bank.npy"), smp.directions())
+                np.save(os.path.join(args.save_inputs, "bank.npy"), bank)
+            # The learned file runs on FP64 projection rounds, as a loaded bank does: bank chains start
+            # on S x = 0 and never reproject, so the rounds' precision changes no result, and with FP64
+            # rounds a chain runs through up to 10 unflagged checks per launch instead of stopping at each
+            # (reduced-precision rounds keep every chain's checks aligned with the rounds).
+            del smp
+            smp = achr.Sampler(model, W, achr.Options(lanes=args.lanes, reproject_mode=2))
+            smp.set_directions(bank)
+            smp.use_directions(True)
+            del bank
         log(f"training run {t_train_run:.1f}s, learn {t_learn:.1f}s: {rep}")
         # 3. timed learned run
         smp.set_run_options(tc, args.trace_thin, args.check_every)
@@ -1053,7 +1071,8 @@ def main():
         meas = PMC_TRAFFIC.get((n, P, args.check_every))
         if meas and out["roofline"]["kernel"] == "step_kernel":
             out["roofline"]["traffic"] = meas[0]
-            out["roofline"]["traffic_note"] = (f"HBM bytes of one {args.check_every}-step launch (FETCH_SIZE x 2.000 + "
+            out["roofline"]["traffic_note"] = (f"HBM bytes of one full launch, {10 * args.check_every} steps (10 "
+                                               f"checks), of the learned step kernel (FETCH_SIZE x 2.000 + "
                                                f"WRITE_SIZE x 1.000, calibrated on a 1 GiB copy); algorithmic "
                                                f"{meas[1]:.4g} B for that launch; {meas[2]}")
         if plain_q is not None:
