@@ -2,10 +2,11 @@
 // as one job.
 //
 // Models whose chains use the same register schedule (threads per chain,
-// slots per thread, direction-bank or centre form) form a bucket; one
-// persistent launch per bucket runs a segment of every chain of every model in
-// it (step_batch_kernel: a grid of resident workgroups walks an item list of
-// {model, first chain}). A segment of the whole batch is every bucket's launch
+// slots per thread, direction-bank or centre form) form a bucket; one launch
+// per bucket runs a segment of every chain of every model in it
+// (step_batch_kernel over an item list of {model, first chain}: one workgroup
+// per item for chains of 128-1,024 lanes, a persistent grid of resident
+// workgroups walking the list for groups of 16-64-lane chains). A segment of the whole batch is every bucket's launch
 // followed by every model's projection round; it is captured once in a HIP
 // graph and replayed for each segment, so a run of S steps costs
 // ceil(S / check_every) + 1 graph launches whatever the number of models.
@@ -99,7 +100,7 @@ int build_buckets(achr_batch* B, const std::vector<int>& which, const std::vecto
     if (e != hipSuccess) return fail(ACHR_E_DEVICE, "step kernel (%d lanes x %d): %s", b.sched.tpc, b.sched.e,
                                      hipGetErrorString(e));
     const int nit = static_cast<int>(b.items.size());
-    b.grid = std::min(nit, b.resident);
+    b.grid = b.sched.cpb == 1 ? nit : std::min(nit, b.resident);  // one workgroup per item, or persistent
     e = b.items_d.upload(b.items.data(), b.items.size(), B->stream);
     if (e != hipSuccess) return fail(ACHR_E_DEVICE, "item upload: %s", hipGetErrorString(e));
   }

@@ -34,9 +34,9 @@ namespace {
 constexpr int kRedSlots = 16;  // wavefront partials per reduction (<= 1024 threads)
 // Largest slot count that prefetches warmup points: at 11 slots (1,024
 // lanes, n = 10,600) the prefetch spills 4 VGPRs and takes 15% off a plain
-// step; 12 and more would spill the point. The batch kernel, which holds
-// its work-item bookkeeping as well, keeps the prefetch to 8 slots (at 10
-// and 11 it would spill 32 and 54 VGPRs).
+// step; 12 and more would spill the point. The persistent batch kernel
+// (several 16-64-lane chains per workgroup), which holds its work-item loop
+// as well, keeps the prefetch to 8 slots.
 constexpr int kPrefetchMaxE = 11, kPrefetchMaxEBatch = 8;
 constexpr int kBoundLookahead = 2;  // slots between a bound load and its use
 // LDS slot arrays of the block-per-chain reductions: the norm, two pairs for
@@ -515,7 +515,8 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
       // Bounds read from memory (slots past the LDS-resident rows) are
       // requested kLook slots ahead of their use (the first ones before the
       // norm's reduction), so their L2 latency overlaps the divisions of the
-      // slots before. The batch kernel, short of registers, reads them at use.
+      // slots before. The persistent batch kernel, short of registers, reads
+      // them at use.
       constexpr int kLook = BATCH ? 0 : kBoundLookahead;
       double2 bq[E];
 #pragma unroll
@@ -796,23 +797,31 @@ __global__ __launch_bounds__(TPC* CPB) void step_kernel(StepArgs a) {
   run_chain<TPC, E, CPB, DIR, false>(a, static_cast<int>(blockIdx.x) * CPB + static_cast<int>(threadIdx.x) / TPC, lds);
 }
 
-// Several models in one persistent launch: the grid is sized to what stays
-// resident, and workgroup w takes items w, w + grid, w + 2*grid, ... of the
-// list, each {model, first chain} naming CPB consecutive chains of one model
-// (the host pads every model to whole items). The item index is uniform over
-// the workgroup, so the model's arguments load into scalar registers. A
-// block-per-chain workgroup passes a barrier between items, so the next
-// chain's first reduction cannot overwrite partials of the last one that a
-// slower wavefront still reads.
+// Several models in one launch, over an item list of {model, first chain},
+// each item naming CPB consecutive chains of one model (the host pads every
+// model to whole items). The item index is uniform over the workgroup, so
+// the model's arguments load into scalar registers. A chain that owns its
+// workgroup (128-1,024 lanes) takes one item per workgroup, and the grid is
+// the item list: a loop over items kept the chain's registers live across
+// iterations and spilled (31 VGPRs at 11 slots), and the chain then runs
+// with the single-model kernel's schedule (warmup point prefetched up to 11
+// slots, bounds requested ahead). Groups of several chains per workgroup
+// (16-64 lanes) run persistently: the grid is sized to what stays resident
+// and workgroup w takes items w, w + grid, w + 2*grid, ...
 template <int TPC, int E, int CPB, bool DIR>
 __global__ __launch_bounds__(TPC* CPB) void step_batch_kernel(const StepArgs* __restrict__ tab,
                                                               const int2* __restrict__ items, int nitems) {
   extern __shared__ double lds[];
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  if constexpr (CPB == 1) {
+    const int it = blockIdx.x;
+    if (it >= nitems) return;
     const int2 w = items[it];
-    const StepArgs a = tab[w.x];
-    run_chain<TPC, E, CPB, DIR, true>(a, w.y + static_cast<int>(threadIdx.x) / TPC, lds);
-    if constexpr (TPC > 64) __syncthreads();
+    run_chain<TPC, E, CPB, DIR, false>(tab[w.x], w.y + static_cast<int>(threadIdx.x) / TPC, lds);
+  } else {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+      const int2 w = items[it];
+      run_chain<TPC, E, CPB, DIR, true>(tab[w.x], w.y + static_cast<int>(threadIdx.x) / TPC, lds);
+    }
   }
 }
 
