@@ -558,8 +558,13 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         // u_i and are corrected to the IEEE quotients (|u_i| > uTol >= 2^-30
         // and both distances > dTol wherever the chord uses them). A bank
         // direction (no reference counterpart) takes the plain products.
+        // A coordinate the chord does not use gets a NaN reciprocal instead:
+        // its candidates are NaN, and v_max_f64 / v_min_f64 return the other
+        // operand for a NaN, so it leaves mn and mx as they are, the same as
+        // skipping it (one 64-bit select per slot instead of two).
         double r_lo, r_hi;
-        const double rcp = rcp_rn(ui);
+        const bool use = du > a.dTol && dl > a.dTol && fabs(ui) > a.uTol;
+        const double rcp = use ? rcp_rn(ui) : __builtin_nan("");
         if constexpr (DIR) {
           r_lo = -dl * rcp;
           r_hi = du * rcp;
@@ -573,11 +578,10 @@ __device__ __forceinline__ void run_chain(const StepArgs a, const int b_in, doub
         // (the reference's two branches pick exactly these). Kept
         // candidates are never NaN, so fmax / fmin equal its compare-and-
         // assign; coordinates that are not kept contribute nothing.
-        const bool use = du > a.dTol && dl > a.dTol && fabs(ui) > a.uTol;
         const double lo = fmin(r_lo, r_hi);
         const double hi = fmax(r_lo, r_hi);
-        mn = vmax(mn, use ? lo : -INFINITY);
-        mx = vmin(mx, use ? hi : INFINITY);
+        mn = vmax(mn, lo);
+        mx = vmin(mx, hi);
         // Schedule four slots at a time: their bound loads overlap, while
         // hoisting every slot's loads and FP64 divisions ahead would multiply
         // the live registers by E.
