@@ -97,12 +97,28 @@ __device__ __forceinline__ void group_sync() {
 }
 
 // A double moved between lanes by a DPP pattern (VALU only, no LDS trip).
+// A lane whose source lies outside its row (row_shl) reads 0 (bound_ctrl):
+// the same +0.0 that a zero "old" operand would leave, without the two moves
+// that set the destination to that old value first.
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, false));
-  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, false));
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, true));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, true));
   return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+// v + (v of lane ^ X) for X = 32 or 16 by the gfx950 lane swaps: after the
+// swap every lane holds its own value and its partner's, one in each of the
+// two outputs, so their sum is the butterfly stage's (addition commutes).
+template <int X>
+__device__ __forceinline__ double xor_sum(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned l = static_cast<unsigned>(b), h = static_cast<unsigned>(b >> 32);
+  auto lo = X == 32 ? __builtin_amdgcn_permlane32_swap(l, l, false, false) : __builtin_amdgcn_permlane16_swap(l, l, false, false);
+  auto hi = X == 32 ? __builtin_amdgcn_permlane32_swap(h, h, false, false) : __builtin_amdgcn_permlane16_swap(h, h, false, false);
+  const double a = __builtin_bit_cast(double, (static_cast<unsigned long long>(hi[0]) << 32) | lo[0]);
+  const double c = __builtin_bit_cast(double, (static_cast<unsigned long long>(hi[1]) << 32) | lo[1]);
+  return a + c;
 }
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
@@ -119,8 +135,8 @@ __device__ __forceinline__ double group_sum(double v, double* red) {
     // lanes below o in each row. Lane 0 depends only on such lanes, so it
     // forms exactly the butterfly's sums, in the same operand order; it is
     // the lane whose value is used (the wavefront's partial, or broadcast).
-    v = v + __shfl_xor(v, 32);
-    v = v + __shfl_xor(v, 16);
+    v = xor_sum<32>(v);
+    v = xor_sum<16>(v);
     v = v + dpp_f64<0x108>(v);
     v = v + dpp_f64<0x104>(v);
     v = v + dpp_f64<0x102>(v);
