@@ -5,6 +5,7 @@
 // UndefinedBehaviorSanitizer on a machine without a GPU (Makefile target
 // `sanitize`, exercised by tests/test_sanitize.py).
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cfloat>
 #include <cmath>
@@ -15,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/achr.h"
@@ -581,15 +583,63 @@ int achr_read_warmup_csv(const char* path, int n_expected, double** warmup, int*
   return achr_host::no_throw([&] { return achr_host::read_warmup_csv(path, n_expected, warmup, W_out, nullptr); });
 }
 
+// The reference's File<k> text (ACHRCuda.cu:744-756): one line per
+// reaction, P "%f" values separated by commas. Rows are formatted by up to
+// 16 threads, each into its own buffer (the same snprintf per value, so the
+// bytes are those of a sequential fprintf), and written in order; rows go in
+// batches so that the buffers hold about 64 MB at most. Where a thread
+// cannot be started its rows are formatted by the calling thread.
 int achr_write_points_csv(const char* path, const double* points, int n, int P) {
   if (!path || !points || n <= 0 || P <= 0) return fail(ACHR_E_ARG, "bad arguments");
   FILE* f = std::fopen(path, "wb");
   if (!f) return fail(ACHR_E_IO, "cannot open %s for writing", path);
-  for (int i = 0; i < n; ++i) {
-    for (int j = 0; j < P - 1; ++j) std::fprintf(f, "%f,", points[static_cast<size_t>(i) * P + j]);
-    std::fprintf(f, "%f\n", points[static_cast<size_t>(i) * P + (P - 1)]);
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int T = static_cast<int>(std::max(1u, std::min(16u, hw)));
+  const size_t per_row = static_cast<size_t>(P) * 16;  // typical "%f," width
+  const int batch = static_cast<int>(
+      std::max<size_t>(static_cast<size_t>(T), std::min<size_t>(static_cast<size_t>(n), (size_t{64} << 20) / per_row)));
+  std::vector<std::string> buf(static_cast<size_t>(T));
+  std::atomic<bool> nomem{false};
+  auto format_rows = [&](int t, int r0, int r1) {
+    try {
+      std::string& out = buf[static_cast<size_t>(t)];
+      out.clear();
+      char tmp[400];  // "%f" of the largest double: 309 digits, sign, point and 6 decimals
+      for (int i = r0; i < r1; ++i) {
+        const double* row = points + static_cast<size_t>(i) * P;
+        for (int j = 0; j < P; ++j) {
+          const int len = std::snprintf(tmp, sizeof(tmp), j + 1 < P ? "%f," : "%f\n", row[j]);
+          out.append(tmp, static_cast<size_t>(len));
+        }
+      }
+    } catch (const std::bad_alloc&) {
+      nomem = true;
+    }
+  };
+  bool ok = true;
+  for (int b0 = 0; b0 < n && ok && !nomem; b0 += batch) {
+    const int b1 = std::min(n, b0 + batch);
+    const int rows = b1 - b0, tn = std::min(T, rows);
+    auto lo = [&](int t) { return b0 + static_cast<int>(static_cast<long long>(rows) * t / tn); };
+    std::vector<std::thread> pool;
+    int started = 1;  // partition 0 is this thread's
+    try {
+      for (int t = 1; t < tn; ++t) {
+        pool.emplace_back(format_rows, t, lo(t), lo(t + 1));
+        ++started;
+      }
+    } catch (const std::exception&) {
+    }
+    for (int t = started; t < tn; ++t) format_rows(t, lo(t), lo(t + 1));
+    format_rows(0, lo(0), lo(1));
+    for (auto& th : pool) th.join();
+    for (int t = 0; t < tn && ok && !nomem; ++t)
+      ok = std::fwrite(buf[static_cast<size_t>(t)].data(), 1, buf[static_cast<size_t>(t)].size(), f) ==
+           buf[static_cast<size_t>(t)].size();
   }
-  if (std::fclose(f) != 0) return fail(ACHR_E_IO, "write error on %s", path);
+  ok = (std::fclose(f) == 0) && ok;
+  if (nomem) return fail(ACHR_E_NOMEM, "out of host memory formatting %s", path);
+  if (!ok) return fail(ACHR_E_IO, "write error on %s", path);
   return ACHR_OK;
 }
 

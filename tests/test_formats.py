@@ -121,6 +121,22 @@ def test_points_csv_writer_layout(tmp_path):
     assert lines == ["1.000000,-2.500000,3.123457", "0.000000,0.000000,-0.000000"]
 
 
+@pytest.mark.parametrize("n,P", [(3, 5), (37, 701), (1021, 64)])
+def test_points_csv_writer_bytes_equal_sequential_printf(tmp_path, n, P):
+    """The writer formats rows on several threads in batches; the file must
+    be byte for byte the sequential "%f" text (the reference's File<k>,
+    ACHRCuda.cu:744-756), whatever the row count against the thread count,
+    with signed zeros, subnormals, huge values and infinities."""
+    rng = np.random.default_rng(n)
+    pts = rng.standard_normal((n, P)) * 10.0 ** rng.integers(-8, 8, (n, P))
+    pts[0, :3] = [-0.0, 5e-324, -1e308][:min(3, P)]
+    pts[-1, -2:] = [np.inf, -np.inf]
+    p = tmp_path / "File0"
+    achr.write_points_csv(str(p), pts)
+    want = "".join(",".join("%f" % v for v in row) + "\n" for row in pts)
+    assert p.read_bytes() == want.encode()
+
+
 def test_model_create_validates():
     with pytest.raises(achr.AchrError):
         achr.Model.from_coo(3, 2, [0, 5], [0, 1], [1.0, 1.0], np.zeros(3), np.ones(3))
