@@ -515,60 +515,116 @@ int achr_model_get_nullspace(const achr_model* M, double* N) {
 
 namespace achr_host {
 
+// The warmup file (ACHRCuda.cu:387-412, :676-689): one line per reaction,
+// the points' values separated by commas. The file is read whole and split
+// into lines (trailing '\r' / '\n' dropped, empty lines skipped, each line
+// cut at its first NUL as a C string would be); the lines are parsed by up to
+// 16 threads, each value by strtod, straight into the column-major array. An
+// error names the first bad line in file order, as a sequential parse would.
 int read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_out, int* n_out) {
   if (!path || !warmup || !W_out) return fail(ACHR_E_ARG, "NULL argument");
   *warmup = nullptr;
-  FILE* f = std::fopen(path, "r");
+  FILE* f = std::fopen(path, "rb");
   if (!f) return fail(ACHR_E_IO, "cannot open warmup CSV %s", path);
   FileGuard guard{f};
-  std::vector<std::vector<double>> rows;
-  std::string line;
-  std::vector<char> buf(1 << 20);
-  int W = -1;
-  int rc = ACHR_OK;
-  while (true) {
-    line.clear();
-    bool got = false;
-    while (std::fgets(buf.data(), static_cast<int>(buf.size()), f)) {
-      got = true;
-      line += buf.data();
-      if (!line.empty() && line.back() == '\n') break;
-    }
-    if (!got) break;
-    while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
-    if (line.empty()) continue;
-    std::vector<double> vals;
-    const char* p = line.c_str();
-    while (true) {
-      char* end = nullptr;
-      const double v = std::strtod(p, &end);
-      if (end == p) { rc = fail(ACHR_E_FORMAT, "%s: line %zu: bad number", path, rows.size() + 1); break; }
-      vals.push_back(v);
-      while (*end == ' ' || *end == '\t') ++end;
-      if (*end == ',') { p = end + 1; continue; }
-      if (*end == '\0') break;
-      rc = fail(ACHR_E_FORMAT, "%s: line %zu: unexpected character", path, rows.size() + 1);
-      break;
-    }
-    if (rc != ACHR_OK) break;
-    if (W < 0) W = static_cast<int>(vals.size());
-    if (static_cast<int>(vals.size()) != W) {
-      rc = fail(ACHR_E_FORMAT, "%s: line %zu has %zu values, expected %d", path, rows.size() + 1, vals.size(), W);
-      break;
-    }
-    rows.push_back(std::move(vals));
+  std::string text;
+  {
+    long size = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) size = std::ftell(f);
+    std::rewind(f);
+    if (size > 0) text.reserve(static_cast<size_t>(size) + 1);
+    std::vector<char> chunk(1 << 20);
+    size_t got = 0;
+    while ((got = std::fread(chunk.data(), 1, chunk.size(), f)) > 0) text.append(chunk.data(), got);
   }
   std::fclose(f);
   f = nullptr;
-  if (rc != ACHR_OK) return rc;
-  const int n = static_cast<int>(rows.size());
-  if (n == 0 || W <= 0) return fail(ACHR_E_FORMAT, "%s: empty warmup file", path);
-  if (n_expected > 0 && n != n_expected)
-    return fail(ACHR_E_FORMAT, "%s: %d lines but the model has %d reactions", path, n, n_expected);
+  // non-empty lines as [begin, end) with text[end] set to '\0'
+  std::vector<std::pair<size_t, size_t>> lines;
+  for (size_t pos = 0; pos < text.size();) {
+    size_t nl = text.find('\n', pos);
+    const size_t stop = nl == std::string::npos ? text.size() : nl;
+    size_t e = stop;
+    if (const void* z = std::memchr(text.data() + pos, '\0', stop - pos))
+      e = static_cast<size_t>(static_cast<const char*>(z) - text.data());
+    while (e > pos && (text[e - 1] == '\r' || text[e - 1] == '\n')) --e;
+    if (e > pos) lines.emplace_back(pos, e);
+    pos = stop + 1;
+  }
+  text.push_back('\0');
+  for (auto& l : lines) text[l.second] = '\0';
+  const int n = static_cast<int>(lines.size());
+  // one line's values: count, and store them at w[k * n + i] for k < W
+  enum { kOk = 0, kBadNumber, kBadChar };
+  auto parse = [&](int i, double* w, int W, int* count) -> int {
+    const char* p = text.c_str() + lines[static_cast<size_t>(i)].first;
+    int k = 0;
+    while (true) {
+      char* end = nullptr;
+      const double v = std::strtod(p, &end);
+      if (end == p) return kBadNumber;
+      if (w && k < W) w[static_cast<size_t>(k) * n + i] = v;
+      ++k;
+      while (*end == ' ' || *end == '\t') ++end;
+      if (*end == ',') { p = end + 1; continue; }
+      if (*end == '\0') break;
+      return kBadChar;
+    }
+    *count = k;
+    return kOk;
+  };
+  auto report = [&](int i, int err, int count, int W) {
+    if (err == kBadNumber) return fail(ACHR_E_FORMAT, "%s: line %d: bad number", path, i + 1);
+    if (err == kBadChar) return fail(ACHR_E_FORMAT, "%s: line %d: unexpected character", path, i + 1);
+    return fail(ACHR_E_FORMAT, "%s: line %d has %d values, expected %d", path, i + 1, count, W);
+  };
+  if (n == 0) return fail(ACHR_E_FORMAT, "%s: empty warmup file", path);
+  int W = 0;
+  if (const int err = parse(0, nullptr, 0, &W)) return report(0, err, 0, 0);
   auto* w = static_cast<double*>(std::malloc(sizeof(double) * static_cast<size_t>(n) * W));
   if (!w) return fail(ACHR_E_NOMEM, "warmup allocation");
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < W; ++k) w[static_cast<size_t>(k) * n + i] = rows[i][k];
+  // lines [lo, hi) per thread; each keeps its first bad line
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int T = n < 64 ? 1 : static_cast<int>(std::max(1u, std::min(16u, hw)));
+  std::vector<int> bad_line(static_cast<size_t>(T), -1), bad_err(static_cast<size_t>(T), 0),
+      bad_count(static_cast<size_t>(T), 0);
+  auto work = [&](int t) {
+    const int lo = static_cast<int>(static_cast<long long>(n) * t / T);
+    const int hi = static_cast<int>(static_cast<long long>(n) * (t + 1) / T);
+    for (int i = lo; i < hi; ++i) {
+      int count = 0;
+      int err = parse(i, w, W, &count);
+      if (err == kOk && count != W) err = -1;
+      if (err) {
+        bad_line[static_cast<size_t>(t)] = i;
+        bad_err[static_cast<size_t>(t)] = err;
+        bad_count[static_cast<size_t>(t)] = count;
+        return;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  int started = 1;
+  try {
+    for (int t = 1; t < T; ++t) {
+      pool.emplace_back(work, t);
+      ++started;
+    }
+  } catch (const std::exception&) {
+  }
+  for (int t = started; t < T; ++t) work(t);
+  work(0);
+  for (auto& th : pool) th.join();
+  for (int t = 0; t < T; ++t) {
+    if (bad_line[static_cast<size_t>(t)] < 0) continue;
+    std::free(w);
+    return report(bad_line[static_cast<size_t>(t)], bad_err[static_cast<size_t>(t)] < 0 ? 0 : bad_err[static_cast<size_t>(t)],
+                  bad_count[static_cast<size_t>(t)], W);
+  }
+  if (n_expected > 0 && n != n_expected) {
+    std::free(w);
+    return fail(ACHR_E_FORMAT, "%s: %d lines but the model has %d reactions", path, n, n_expected);
+  }
   *warmup = w;
   *W_out = W;
   if (n_out) *n_out = n;

@@ -113,6 +113,29 @@ def test_warmup_csv_errors(tmp_path):
         achr.read_warmup_csv(str(p), 3)  # reaction count mismatch
 
 
+def test_warmup_csv_reader_in_parallel_and_its_first_error(tmp_path):
+    """Files of many lines are parsed by several threads: the values equal
+    numpy's parse bit for bit (empty lines, CR LF and blanks around values
+    as the reference's reader takes them), and of several bad lines spread
+    over the threads' blocks the first in file order is reported."""
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((700, 37)) * 10.0 ** rng.integers(-6, 6, (700, 37))
+    lines = [",".join(repr(float(v)) for v in r) for r in X]
+    lines[5] = " " + lines[5].replace(",", " ,\t", 3)
+    p = tmp_path / "w.csv"
+    p.write_bytes(("\r\n".join(lines[:300]) + "\n\n" + "\n".join(lines[300:]) + "\n").encode())
+    W = achr.read_warmup_csv(str(p), 700)
+    assert W.view(np.uint64).tolist() == X.view(np.uint64).tolist()
+    bad = list(lines)
+    bad[650] = bad[650] + ",oops"
+    bad[420] = bad[420].rsplit(",", 1)[0]
+    bad[444] = "x" + bad[444]
+    p.write_text("\n".join(bad) + "\n")
+    with pytest.raises(achr.AchrError) as e:
+        achr.read_warmup_csv(str(p), 700)
+    assert "line 421 has 36 values, expected 37" in str(e.value)
+
+
 def test_points_csv_writer_layout(tmp_path):
     pts = np.array([[1.0, -2.5, 3.1234567], [0.0, 1e-7, -0.0000004]])
     p = tmp_path / "File0"
