@@ -824,7 +824,13 @@ def run_putida(args):
             raise RuntimeError(r.stderr)
         pts = achr.read_points_bin(os.path.join(work, "File0.bin"))
         lanes_line = [l for l in r.stdout.splitlines() if "threads per chain" in l]
+
+        def cli_seconds(tag):
+            # the command line's own clock: "<tag> done in X seconds." from its start
+            hit = [l for l in r.stdout.splitlines() if l.startswith(tag + " done in")]
+            return float(hit[0].split()[-2]) if hit else None
         runs[P] = {"points": P, "steps": 1000, "wall_s": wall, "chain_steps_per_s": P * 1000 / wall,
+                   "cli_setup_s": cli_seconds("Null space"), "cli_total_s": cli_seconds("Sampling"),
                    "drift_inf": float(np.abs(Ssp @ pts).max()),
                    "in_bounds": bool((pts >= lb[:, None] - 1e-12).all() and (pts <= ub[:, None] + 1e-12).all()),
                    "cli": lanes_line[0] if lanes_line else ""}
