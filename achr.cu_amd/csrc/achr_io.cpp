@@ -555,7 +555,7 @@ int read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_ou
   for (auto& l : lines) text[l.second] = '\0';
   const int n = static_cast<int>(lines.size());
   // one line's values: count, and store them at w[k * n + i] for k < W
-  enum { kOk = 0, kBadNumber, kBadChar };
+  enum { kOk = 0, kBadNumber, kBadChar, kBadCount };
   auto parse = [&](int i, double* w, int W, int* count) -> int {
     const char* p = text.c_str() + lines[static_cast<size_t>(i)].first;
     int k = 0;
@@ -580,7 +580,7 @@ int read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_ou
   };
   if (n == 0) return fail(ACHR_E_FORMAT, "%s: empty warmup file", path);
   int W = 0;
-  if (const int err = parse(0, nullptr, 0, &W)) return report(0, err, 0, 0);
+  if (const int err = parse(0, nullptr, 0, &W)) return report(0, err, 0, 0);  // never kBadCount
   auto* w = static_cast<double*>(std::malloc(sizeof(double) * static_cast<size_t>(n) * W));
   if (!w) return fail(ACHR_E_NOMEM, "warmup allocation");
   // lines [lo, hi) per thread; each keeps its first bad line
@@ -594,7 +594,7 @@ int read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_ou
     for (int i = lo; i < hi; ++i) {
       int count = 0;
       int err = parse(i, w, W, &count);
-      if (err == kOk && count != W) err = -1;
+      if (err == kOk && count != W) err = kBadCount;
       if (err) {
         bad_line[static_cast<size_t>(t)] = i;
         bad_err[static_cast<size_t>(t)] = err;
@@ -618,8 +618,8 @@ int read_warmup_csv(const char* path, int n_expected, double** warmup, int* W_ou
   for (int t = 0; t < T; ++t) {
     if (bad_line[static_cast<size_t>(t)] < 0) continue;
     std::free(w);
-    return report(bad_line[static_cast<size_t>(t)], bad_err[static_cast<size_t>(t)] < 0 ? 0 : bad_err[static_cast<size_t>(t)],
-                  bad_count[static_cast<size_t>(t)], W);
+    const size_t u = static_cast<size_t>(t);
+    return report(bad_line[u], bad_err[u], bad_count[u], W);
   }
   if (n_expected > 0 && n != n_expected) {
     std::free(w);

@@ -103,8 +103,9 @@ __device__ __forceinline__ void group_sync() {
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, true));
-  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, true));
+  const int l = static_cast<int>(b), h = static_cast<int>(b >> 32);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, l, CTRL, 0xF, 0xF, true));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, h, CTRL, 0xF, 0xF, true));
   return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 // v + (v of lane ^ X) for X = 32 or 16 by the gfx950 lane swaps: after the
@@ -114,8 +115,11 @@ template <int X>
 __device__ __forceinline__ double xor_sum(double v) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
   const unsigned l = static_cast<unsigned>(b), h = static_cast<unsigned>(b >> 32);
-  auto lo = X == 32 ? __builtin_amdgcn_permlane32_swap(l, l, false, false) : __builtin_amdgcn_permlane16_swap(l, l, false, false);
-  auto hi = X == 32 ? __builtin_amdgcn_permlane32_swap(h, h, false, false) : __builtin_amdgcn_permlane16_swap(h, h, false, false);
+  auto swap = [](unsigned u) {
+    return X == 32 ? __builtin_amdgcn_permlane32_swap(u, u, false, false)
+                   : __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  };
+  const auto lo = swap(l), hi = swap(h);
   const double a = __builtin_bit_cast(double, (static_cast<unsigned long long>(hi[0]) << 32) | lo[0]);
   const double c = __builtin_bit_cast(double, (static_cast<unsigned long long>(hi[1]) << 32) | lo[1]);
   return a + c;
@@ -130,11 +134,12 @@ template <int TPC>
 __device__ __forceinline__ double group_sum(double v, double* red) {
   constexpr int G = TPC < 64 ? TPC : 64;
   if constexpr (G == 64) {
-    // The butterfly's stages 32 and 16 as shuffles; stages 8, 4, 2 and 1 by
-    // row_shl:o, which hands lane i the value of lane i + o = i ^ o for the
-    // lanes below o in each row. Lane 0 depends only on such lanes, so it
-    // forms exactly the butterfly's sums, in the same operand order; it is
-    // the lane whose value is used (the wavefront's partial, or broadcast).
+    // The butterfly's stages 32 and 16 by lane swaps (each lane's sum of its
+    // value and its partner's); stages 8, 4, 2 and 1 by row_shl:o, which
+    // hands lane i the value of lane i + o = i ^ o for the lanes below o in
+    // each row. Lane 0 depends only on such lanes, so it forms exactly the
+    // butterfly's sums (addition commutes); it is the lane whose value is
+    // used (the wavefront's partial, or broadcast).
     v = xor_sum<32>(v);
     v = xor_sum<16>(v);
     v = v + dpp_f64<0x108>(v);
