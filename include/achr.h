@@ -287,7 +287,8 @@ int achr_sampler_get_warmup(const achr_sampler *sampler, double *warmup, int *W)
 
 /* Several samplers (one per model, any mix of reaction counts) run as one
  * job. Chains of the models that share a register schedule (threads per
- * chain, slots per thread) run in one persistent launch per segment; a
+ * chain, slots per thread) run in one launch per segment (one workgroup per
+ * chain of 128-1,024 threads, persistent workgroups for smaller chains); a
  * segment of the whole batch (every such launch, then every model's
  * projection round) is captured once in a HIP graph and replayed, so a run
  * costs ceil(S / check_every) + 1 graph launches however many models it
@@ -405,7 +406,8 @@ int achr_sampler_start_point(const achr_sampler *sampler, double *x0);
 /* -------------------------------------------------------------- file I/O */
 
 /* Writes File<k>-style text: n lines of P comma-separated "%f" values
- * (ACHRCuda.cu:744-756). */
+ * (ACHRCuda.cu:744-756). Rows are formatted by up to 16 host threads; the
+ * bytes are those of a sequential fprintf. */
 int achr_write_points_csv(const char *path, const double *points, int n, int P);
 
 /* ------------------------------------------------------- warmup points */
@@ -457,7 +459,8 @@ int achr_write_points_bin(const char *path, const double *points, int n, int P);
 int achr_read_points_bin(const char *path, double **points, int *n, int *P);
 
 /* Reads a warmup CSV into a newly malloc'ed n x W column-major array
- * (release with achr_free). */
+ * (release with achr_free). The file is read whole and its lines parsed by
+ * up to 16 host threads; an error names the first bad line. */
 int achr_read_warmup_csv(const char *path, int n_expected, double **warmup, int *W);
 
 void achr_free(void *p);
