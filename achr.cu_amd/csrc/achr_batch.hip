@@ -58,6 +58,11 @@ struct achr_batch {
   DevBuf<achr::StepArgs> tab_d;
   std::vector<Bucket> buckets;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  // branches of a segment: bucket b and its models' projection rounds run
+  // on aux[b % aux.size()], forked from and joined back into `stream`
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> ev_join;
+  hipEvent_t ev_fork = nullptr;
   int segments = 0;
   int graphs = 0;
   size_t graph_nodes = 0;
@@ -164,6 +169,10 @@ int achr_batch_create(achr_sampler* const* samplers, int count, achr_batch** out
 void achr_batch_destroy(achr_batch* B) {
   if (!B) return;
   if (B->stream) (void)hipStreamSynchronize(B->stream);
+  for (hipStream_t a : B->aux) (void)hipStreamSynchronize(a);
+  for (hipStream_t a : B->aux) (void)hipStreamDestroy(a);
+  for (hipEvent_t e : B->ev_join) (void)hipEventDestroy(e);
+  if (B->ev_fork) (void)hipEventDestroy(B->ev_fork);
   if (B->ev_begin) (void)hipEventDestroy(B->ev_begin);
   if (B->ev_end) (void)hipEventDestroy(B->ev_end);
   if (B->own_stream && B->stream) (void)hipStreamDestroy(B->stream);
@@ -295,9 +304,41 @@ int achr_batch_sample(achr_batch* B, const int* P, int Ssteps, uint64_t seed, ui
       sig[k] = static_cast<char>(seg + 1 < segs[k] ? achr_host::projection_kind(B->s[k], seg) : achr_host::kProjNone);
     return sig;
   };
+  // Branch streams (at most the four hardware queues a process gets),
+  // created once per batch.
+  const int nbr = std::min<int>(4, static_cast<int>(B->buckets.size()));
+  while (static_cast<int>(B->aux.size()) < nbr) {
+    hipStream_t a = nullptr;
+    hipEvent_t e = nullptr;
+    hipError_t he = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (he != hipSuccess) return fail(ACHR_E_DEVICE, "batch branch stream: %s", hipGetErrorString(he));
+    B->aux.push_back(a);
+    B->ev_join.push_back(e);
+  }
+  if (!B->ev_fork) HIP_TRY(hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming));
+  // A segment: every bucket's launch and then its models' projection
+  // rounds, one branch per bucket (a model's round depends only on its own
+  // chains), the branches forked from and joined back into the batch stream.
   auto enqueue_segment = [&](const std::string& sig) {
-    int r = launch_buckets(B, B->buckets);
-    for (int k = 0; k < count && r == ACHR_OK; ++k) r = achr_host::enqueue_projection(B->s[k], sig[k], B->stream);
+    hipError_t he = hipEventRecord(B->ev_fork, B->stream);
+    for (int a = 0; a < nbr && he == hipSuccess; ++a) he = hipStreamWaitEvent(B->aux[a], B->ev_fork, 0);
+    if (he != hipSuccess) return fail(ACHR_E_DEVICE, "batch fork: %s", hipGetErrorString(he));
+    int r = ACHR_OK;
+    for (size_t b = 0; b < B->buckets.size() && r == ACHR_OK; ++b) {
+      const Bucket& bk = B->buckets[b];
+      hipStream_t st = B->aux[b % nbr];
+      he = achr::launch_step_batch(B->tab_d.p, bk.items_d.p, static_cast<int>(bk.items.size()), bk.grid, bk.sched,
+                                   bk.dir, st);
+      if (he != hipSuccess) r = fail(ACHR_E_DEVICE, "batched step launch: %s", hipGetErrorString(he));
+      for (size_t j = 0; j < bk.models.size() && r == ACHR_OK; ++j)
+        r = achr_host::enqueue_projection(B->s[bk.models[j]], sig[bk.models[j]], st);
+    }
+    for (int a = 0; a < nbr; ++a) {
+      he = hipEventRecord(B->ev_join[a], B->aux[a]);
+      if (he == hipSuccess) he = hipStreamWaitEvent(B->stream, B->ev_join[a], 0);
+      if (he != hipSuccess && r == ACHR_OK) r = fail(ACHR_E_DEVICE, "batch join: %s", hipGetErrorString(he));
+    }
     return r;
   };
   std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
