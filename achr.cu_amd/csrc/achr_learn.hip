@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "achr_dirnet.h"
@@ -437,19 +438,47 @@ static int free_basis(const achr_model* M, const double* dN, DevBuf<double>& Nf,
   return rc;
 }
 
+// Runs f(t, T) on T host threads (at most 16; the calling thread is one of
+// them, and it also takes the share of any thread that cannot be started).
+// Each f writes its own part of the outputs, so the results do not depend on
+// the thread count.
+extern "C++" {
+template <class F>
+static void on_threads(F&& f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int T = static_cast<int>(std::max(1u, std::min(16u, hw)));
+  std::vector<std::thread> pool;
+  int started = 1;
+  try {
+    for (int t = 1; t < T; ++t) {
+      pool.emplace_back(f, t, T);
+      ++started;
+    }
+  } catch (const std::exception&) {
+  }
+  for (int t = started; t < T; ++t) f(t, T);
+  f(0, T);
+  for (auto& th : pool) th.join();
+}
+}  // extern "C++"
+
 // Cholesky-QR (twice) of the df x q FP64 matrix G (column major, host): G <- Q.
+// The products run on host threads, each element summed in the sequential
+// order (columns of A dealt cyclically, G updated by row blocks).
 static bool chol_qr(std::vector<double>& G, int rows, int q) {
   for (int pass = 0; pass < 2; ++pass) {
     std::vector<double> A(static_cast<size_t>(q) * q, 0.0);
-    for (int a = 0; a < q; ++a)
-      for (int b = a; b < q; ++b) {
-        const double* ga = G.data() + static_cast<size_t>(a) * rows;
-        const double* gb = G.data() + static_cast<size_t>(b) * rows;
-        double s = 0.0;
-        for (int i = 0; i < rows; ++i) s += ga[i] * gb[i];
-        A[a + static_cast<size_t>(b) * q] = s;
-        A[b + static_cast<size_t>(a) * q] = s;
-      }
+    on_threads([&](int t, int T) {
+      for (int a = t; a < q; a += T)
+        for (int b = a; b < q; ++b) {
+          const double* ga = G.data() + static_cast<size_t>(a) * rows;
+          const double* gb = G.data() + static_cast<size_t>(b) * rows;
+          double s = 0.0;
+          for (int i = 0; i < rows; ++i) s += ga[i] * gb[i];
+          A[a + static_cast<size_t>(b) * q] = s;
+          A[b + static_cast<size_t>(a) * q] = s;
+        }
+    });
     // A = R^T R (upper R)
     std::vector<double> R(static_cast<size_t>(q) * q, 0.0);
     for (int j = 0; j < q; ++j) {
@@ -464,17 +493,21 @@ static bool chol_qr(std::vector<double>& G, int rows, int q) {
         R[j + static_cast<size_t>(i) * q] = t / rjj;
       }
     }
-    // G <- G R^{-1}, column by column
-    for (int j = 0; j < q; ++j) {
-      double* gj = G.data() + static_cast<size_t>(j) * rows;
-      for (int k = 0; k < j; ++k) {
-        const double r = R[k + static_cast<size_t>(j) * q];
-        const double* gk = G.data() + static_cast<size_t>(k) * rows;
-        for (int i = 0; i < rows; ++i) gj[i] -= r * gk[i];
+    // G <- G R^{-1}, column by column within each thread's block of rows
+    on_threads([&](int t, int T) {
+      const int i0 = static_cast<int>(static_cast<long long>(rows) * t / T);
+      const int i1 = static_cast<int>(static_cast<long long>(rows) * (t + 1) / T);
+      for (int j = 0; j < q; ++j) {
+        double* gj = G.data() + static_cast<size_t>(j) * rows;
+        for (int k = 0; k < j; ++k) {
+          const double r = R[k + static_cast<size_t>(j) * q];
+          const double* gk = G.data() + static_cast<size_t>(k) * rows;
+          for (int i = i0; i < i1; ++i) gj[i] -= r * gk[i];
+        }
+        const double inv = 1.0 / R[j + static_cast<size_t>(j) * q];
+        for (int i = i0; i < i1; ++i) gj[i] *= inv;
       }
-      const double inv = 1.0 / R[j + static_cast<size_t>(j) * q];
-      for (int i = 0; i < rows; ++i) gj[i] *= inv;
-    }
+    });
   }
   return true;
 }
@@ -639,15 +672,25 @@ int achr_sampler_learn_directions(achr_sampler* S, const achr_learn_options* opt
       std::vector<double> lam(r);
       captured = 0.0;
       for (int j = 0; j < r; ++j) {
-        const int src = ord[j];
-        lam[j] = std::max(0.0, A[src + static_cast<size_t>(src) * q]);
+        lam[j] = std::max(0.0, A[ord[j] + static_cast<size_t>(ord[j]) * q]);
         captured += lam[j];
-        for (int i = 0; i < df; ++i) {
-          double s = 0.0;
-          for (int k = 0; k < q; ++k) s += G[static_cast<size_t>(i) + static_cast<size_t>(k) * df] * V[k + static_cast<size_t>(src) * q];
-          Qr[static_cast<size_t>(i) + static_cast<size_t>(j) * df] = static_cast<float>(s);
-        }
       }
+      // Qr[:, j] = G V[:, ord[j]], each element summed over k in order; the
+      // k loop outside the i loop reads G by columns (host threads by j)
+      on_threads([&](int t, int T) {
+        std::vector<double> acc(static_cast<size_t>(df));
+        for (int j = t; j < r; j += T) {
+          const double* vcol = V.data() + static_cast<size_t>(ord[j]) * q;
+          std::fill(acc.begin(), acc.end(), 0.0);
+          for (int k = 0; k < q; ++k) {
+            const double* gk = G.data() + static_cast<size_t>(k) * df;
+            const double vk = vcol[k];
+            for (int i = 0; i < df; ++i) acc[static_cast<size_t>(i)] += gk[i] * vk;
+          }
+          float* qcol = Qr.data() + static_cast<size_t>(j) * df;
+          for (int i = 0; i < df; ++i) qcol[i] = static_cast<float>(acc[static_cast<size_t>(i)]);
+        }
+      });
       sig2 = df > r ? std::max(tot - captured, 0.0) / (df - r) : 0.0;
       captured = tot > 0.0 ? captured / tot : 0.0;
       // The network emits whitened directions (target: identity second
