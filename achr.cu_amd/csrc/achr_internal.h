@@ -127,8 +127,10 @@ hipError_t launch_dgemm_cols(int M, int K, const double* A, long sar, long sak, 
                              double* C, long ldc, const int* count, int max_cols, hipStream_t stream);
 // Normalises each of the K columns (length n) of A to unit Euclidean norm.
 hipError_t launch_normalize_cols(double* A, int n, int K, hipStream_t stream);
-// Column means of the rows x cols row-major FP64 matrix X (mean[j]).
-hipError_t launch_col_means(const double* X, long rows, int cols, double* mean, hipStream_t stream);
+// Column means of the rows x cols row-major FP64 matrix X (mean[j]), summed
+// over kColMeanStripes row stripes; work holds kColMeanStripes * cols doubles.
+constexpr int kColMeanStripes = 64;
+hipError_t launch_col_means(const double* X, long rows, int cols, double* mean, double* work, hipStream_t stream);
 // Y[r, j] -= v[j] for the rows x cols row-major FP32 matrix Y.
 hipError_t launch_sub_row(float* Y, long rows, int cols, const float* v, hipStream_t stream);
 // Standard normal FP32 values from the Philox stream.

@@ -572,13 +572,14 @@ int achr_sampler_learn_directions(achr_sampler* S, const achr_learn_options* opt
     HIP_TRY(hipStreamSynchronize(S->stream));
     // Samples in free null-space coordinates: Y = (X - mean) N_f, M x df FP32.
     DevBuf<float> Nf32, Y, mf;
-    DevBuf<double> mean;
+    DevBuf<double> mean, mean_work;
     HIP_TRY(Nf32.ensure(static_cast<size_t>(n) * df));
     HIP_TRY(achr::launch_to_f32(Nf.p, Nf32.p, static_cast<size_t>(n) * df, nullptr));
     HIP_TRY(Y.ensure(static_cast<size_t>(Msamp) * df));
     HIP_TRY(mean.ensure(n));
     HIP_TRY(mf.ensure(df));
-    HIP_TRY(achr::launch_col_means(S->trace.p, Msamp, n, mean.p, nullptr));
+    HIP_TRY(mean_work.ensure(static_cast<size_t>(achr::kColMeanStripes) * n));
+    HIP_TRY(achr::launch_col_means(S->trace.p, Msamp, n, mean.p, mean_work.p, nullptr));
     {
       achr::GemmArgs g = gemm(static_cast<int>(Msamp), df, n, S->trace.p, n, 1, Nf32.p, 1, n, Y.p, df, 1);
       g.a_f64 = 1;

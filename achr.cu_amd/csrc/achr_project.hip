@@ -248,11 +248,26 @@ __global__ __launch_bounds__(256) void normalize_cols_kernel(double* __restrict_
     for (int i = threadIdx.x; i < n; i += 256) col[i] = col[i] / nrm;
 }
 
-__global__ void col_means_kernel(const double* __restrict__ X, long rows, int cols, double* __restrict__ mean) {
+// Column sums of one stripe of rows (blockIdx.y of gridDim.y equal stripes),
+// each in row order; col_means_finish_kernel adds the stripes in stripe order,
+// so the mean is the same on every run.
+__global__ void col_partial_kernel(const double* __restrict__ X, long rows, int cols, double* __restrict__ part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  const long per = (rows + gridDim.y - 1) / gridDim.y;
+  const long r0 = static_cast<long>(blockIdx.y) * per;
+  const long r1 = r0 + per < rows ? r0 + per : rows;
+  double s = 0.0;
+  for (long r = r0; r < r1; ++r) s += X[r * cols + j];
+  part[static_cast<size_t>(blockIdx.y) * cols + j] = s;
+}
+
+__global__ void col_means_finish_kernel(const double* __restrict__ part, int stripes, long rows, int cols,
+                                        double* __restrict__ mean) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
   double s = 0.0;
-  for (long r = 0; r < rows; ++r) s += X[r * cols + j];
+  for (int k = 0; k < stripes; ++k) s += part[static_cast<size_t>(k) * cols + j];
   mean[j] = s / static_cast<double>(rows);
 }
 
@@ -327,8 +342,11 @@ hipError_t launch_normalize_cols(double* A, int n, int K, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_col_means(const double* X, long rows, int cols, double* mean, hipStream_t stream) {
-  hipLaunchKernelGGL(col_means_kernel, dim3((cols + 255) / 256), dim3(256), 0, stream, X, rows, cols, mean);
+hipError_t launch_col_means(const double* X, long rows, int cols, double* mean, double* work, hipStream_t stream) {
+  if (cols <= 0) return hipSuccess;
+  const dim3 grid((cols + 255) / 256);
+  hipLaunchKernelGGL(col_partial_kernel, dim3(grid.x, kColMeanStripes), dim3(256), 0, stream, X, rows, cols, work);
+  hipLaunchKernelGGL(col_means_finish_kernel, grid, dim3(256), 0, stream, work, kColMeanStripes, rows, cols, mean);
   return hipGetLastError();
 }
 
