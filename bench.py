@@ -901,6 +901,9 @@ def main():
     ap.add_argument("--fp64-every", type=int, default=10)
     ap.add_argument("--train-steps", type=int, default=1000, help="plain-ACHR steps that produce the training set")
     ap.add_argument("--train-thin", type=int, default=100)
+    ap.add_argument("--train-fp64-every", type=int, default=0,
+                    help="every n-th projection round of the training run in FP64 (0: none; the residual-form "
+                         "rounds recompute S x in FP64 each time, so they do not let the residual grow)")
     ap.add_argument("--rank", type=int, default=512)
     ap.add_argument("--iters", type=int, default=400)
     ap.add_argument("--check-every", type=int, default=100, help="residual check period of the learned sampler")
@@ -1033,15 +1036,20 @@ def main():
             # 1. training set: plain ACHR (residual BF16x3 projection) with every chain traced
             t = time.time()
             topts = achr.Options(lanes=args.lanes, trace_chains=P, trace_thin=args.train_thin,
-                                 proj_prec=PROJ["res_bf16x3"], reproject_mode=2)
+                                 proj_prec=PROJ["res_bf16x3"], reproject_mode=2,
+                                 fp64_every=args.train_fp64_every if args.train_fp64_every > 0 else 1 << 30)
             smp = achr.Sampler(model, W, topts)
             smp.run(P, args.train_steps, seed=99)
             smp.timing()
             t_train_run = time.time() - t
+            import scipy.sparse as sp
+            train_drift = float(np.abs(sp.csr_matrix((m.vals, (m.rows, m.cols)), shape=(m.m, m.n))
+                                       @ smp.points()[:, :1024]).max())
             # 2. rounding map + network + direction bank
             t = time.time()
             rep = smp.learn_directions(rank=args.rank, width=512, iters=args.iters, batch=2048, lr=1e-3, seed=7)
             t_learn = time.time() - t
+            rep["train_drift_inf"] = train_drift
             bank = smp.directions()
             if args.save_inputs:
                 np.save(os.path.join(args.save_inputs, "bank.npy"), bank)
@@ -1069,7 +1077,8 @@ def main():
         out["config"] = cfg
         out.update({"chain_steps_per_s": P * S / wall, "min_ess_traced": q["min_ess"], "traced_chains": q["traced"],
                     "drift_inf": q["drift"], "max_rhat": q["max_rhat"],
-                    "learn": {"train_run_s": t_train_run, "learn_s": t_learn, **rep},
+                    "learn": {"train_run_s": t_train_run, "learn_s": t_learn,
+                              "train_fp64_every": args.train_fp64_every, **rep},
                     "kernel_ms": tm})
         out["roofline"] = roofline_for("fp64", m, n, d, P, S, q, tm, bank=True)
         out["roofline"]["measured_copy_GBs"] = measure_copy_gbs()
