@@ -291,17 +291,37 @@ static int sampler_create(const achr_model* M, const double* warmup, int W, cons
   const double nd = static_cast<double>(M->n) * static_cast<double>(M->d);
   S->mode = o.reproject_mode != 0 ? o.reproject_mode : (nd <= 65536.0 && o.proj_prec == ACHR_PROJ_FP64 ? 1 : 2);
   // Warmup centre on the host: (1/W) * sum_k w_k, k ascending (createCenterPt).
+  // Host threads take ranges of coordinates and add the points one after the
+  // other (contiguous reads), so each coordinate is still summed from +0.0 in
+  // k order.
   const int n = M->n;
   S->c0.assign(n, 0.0);
   const double alpha = 1.0 / static_cast<double>(W);
-  for (int i = 0; i < n; ++i) {
-    double s = 0.0;
-    for (int k = 0; k < W; ++k) s += warmup[static_cast<size_t>(k) * n + i];
-    S->c0[i] = alpha * s;
-  }
-  std::vector<double> Nrm(static_cast<size_t>(n) * M->d);
-  for (int j = 0; j < M->d; ++j)
-    for (int i = 0; i < n; ++i) Nrm[static_cast<size_t>(i) * M->d + j] = M->N[static_cast<size_t>(j) * n + i];
+  achr_host::on_threads([&](int t, int T) {
+    const int i0 = static_cast<int>(static_cast<long>(n) * t / T);
+    const int i1 = static_cast<int>(static_cast<long>(n) * (t + 1) / T);
+    double* c = S->c0.data();
+    for (int k = 0; k < W; ++k) {
+      const double* wk = warmup + static_cast<size_t>(k) * n;
+      for (int i = i0; i < i1; ++i) c[i] += wk[i];
+    }
+    for (int i = i0; i < i1; ++i) c[i] = alpha * c[i];
+  });
+  // N row major (n x d), transposed in 64 x 64 tiles; threads take 64-row bands.
+  const int d = M->d;
+  std::vector<double> Nrm(static_cast<size_t>(n) * std::max(d, 0));
+  achr_host::on_threads([&](int t, int T) {
+    for (int ib = 64 * t; ib < n; ib += 64 * T) {
+      const int ie = std::min(n, ib + 64);
+      for (int jb = 0; jb < d; jb += 64) {
+        const int je = std::min(d, jb + 64);
+        for (int j = jb; j < je; ++j) {
+          const double* src = M->N.data() + static_cast<size_t>(j) * n;
+          for (int i = ib; i < ie; ++i) Nrm[static_cast<size_t>(i) * d + j] = src[i];
+        }
+      }
+    }
+  });
   // S as a sliced ELL table for the residual check of chains of 64 or more
   // lanes: rows sorted by length (longest first), 64 rows to a slice, each
   // slice as wide as its longest row, entries of a row in CSR order.

@@ -6,7 +6,9 @@
 #include <cstdio>
 #include <string>
 #include <new>
+#include <algorithm>
 #include <stdexcept>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -166,5 +168,26 @@ int enqueue_projection(achr_sampler* S, int kind, hipStream_t st);
 // Records the settings of the run just queued (directions, check period,
 // trace, schedule) that achr_sample_continue and the checkpoint must match.
 void note_run(achr_sampler* S, const achr::Schedule& sc);
+
+// Runs f(t, T) for t = 0 .. T-1, T = min(16, hardware threads), on host
+// threads (t = 0 on the caller); a part whose thread cannot be started runs
+// on the caller.
+template <class F>
+void on_threads(F&& f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int T = static_cast<int>(std::max(1u, std::min(16u, hw)));
+  std::vector<std::thread> pool;
+  int started = 1;
+  try {
+    for (int t = 1; t < T; ++t) {
+      pool.emplace_back(f, t, T);
+      ++started;
+    }
+  } catch (const std::exception&) {
+  }
+  for (int t = started; t < T; ++t) f(t, T);
+  f(0, T);
+  for (auto& th : pool) th.join();
+}
 
 }  // namespace achr_host

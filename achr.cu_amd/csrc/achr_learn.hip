@@ -19,6 +19,7 @@
 
 using achr_host::DevBuf;
 using achr_host::fail;
+using achr_host::on_threads;
 
 struct achr_dirnet {
   achr::DirNet net;
@@ -442,25 +443,6 @@ static int free_basis(const achr_model* M, const double* dN, DevBuf<double>& Nf,
 // them, and it also takes the share of any thread that cannot be started).
 // Each f writes its own part of the outputs, so the results do not depend on
 // the thread count.
-extern "C++" {
-template <class F>
-static void on_threads(F&& f) {
-  const unsigned hw = std::thread::hardware_concurrency();
-  const int T = static_cast<int>(std::max(1u, std::min(16u, hw)));
-  std::vector<std::thread> pool;
-  int started = 1;
-  try {
-    for (int t = 1; t < T; ++t) {
-      pool.emplace_back(f, t, T);
-      ++started;
-    }
-  } catch (const std::exception&) {
-  }
-  for (int t = started; t < T; ++t) f(t, T);
-  f(0, T);
-  for (auto& th : pool) th.join();
-}
-}  // extern "C++"
 
 // Cholesky-QR (twice) of the df x q FP64 matrix G (column major, host): G <- Q.
 // The products run on host threads, each element summed in the sequential
