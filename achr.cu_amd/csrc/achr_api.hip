@@ -132,18 +132,121 @@ int achr_model_compute_nullspace(achr_model* M, int* null_dim) {
 
 // ------------------------------------------------------ pseudo-inverse S+
 
-// S+ = S^T (S S^T)+ with the eigen-decomposition of S S^T (rocSOLVER syevd,
-// eigenvalues below m * lambda_max * 2.2204e-16 dropped, the rule the
-// reference applies to S^T S), (S S^T)+ by one rocBLAS DGEMM, then S^T times
-// it straight from the sparse columns of S (launch_pinv_csc); stored as FP32.
-// S itself is never made dense. x - S+ (S x) is the orthogonal projection of
-// x onto null(S), the same operator as N N^T x.
-static int build_pinv_f32(const achr_model* M, DevBuf<float>& out) {
+// build_pinv_f32 stores S+ as FP32, n x m column major: by pinv_chol when it
+// succeeds, else by the eigen route, S+ = S^T (S S^T)+ with the
+// eigen-decomposition of S S^T (rocSOLVER syevd, eigenvalues below
+// m * lambda_max * 2.2204e-16 dropped, the rule the reference applies to
+// S^T S), (S S^T)+ by one rocBLAS DGEMM, then S^T times it straight from the
+// sparse columns of S (launch_pinv_csc; S is never made dense on this route).
+// x - S+ (S x) is the orthogonal projection of x onto null(S), the same
+// operator as N N^T x.
+// With the null-space basis N (n x d, on the device as dN) S+ needs no
+// eigen-decomposition: G = S^T S + sigma N N^T is symmetric positive definite
+// when N spans null(S) (eigenvalues: those of S^T S on the row space of S,
+// sigma on null(S); sigma, the largest diagonal entry of S^T S, keeps both
+// ranges alike), and since N N^T S^T = 0, G^-1 S^T = S+. One Cholesky
+// factorisation and two triangular solves against the dense S^T (built on the
+// device only, then overwritten by S+). Returns false, and the caller takes
+// the eigen route, when the factorisation fails, or when for r = S z (z a
+// fixed pseudo-random vector) S+ r, which is z's part in the row space of S,
+// misses z - N (N^T z) by more than 1e-7 of max |z|: N must be an orthonormal
+// basis of all of null(S) (a direction missing from N leaves G singular
+// there, and its roundoff-sized pivot puts garbage along it).
+static bool pinv_chol(const achr_model* M, const double* dN, DevBuf<float>& out) {
+  const int n = M->n, m = M->m, d = M->d;
+  double sigma = 0.0;
+  {
+    std::vector<double> cn(n, 0.0);
+    for (size_t p = 0; p < M->vals.size(); ++p) cn[M->colidx[p]] += M->vals[p] * M->vals[p];
+    for (double v : cn) sigma = std::max(sigma, v);
+  }
+  if (!(sigma > 0.0) || (d > 0 && !dN)) return false;
+  // check vector: r = S z, t = S^T r
+  std::vector<double> z(n), r(m, 0.0), t(n, 0.0), y(n);
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  for (int i = 0; i < n; ++i) {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    z[i] = static_cast<double>(st >> 11) * 0x1.0p-52 - 1.0;
+  }
+  double rmax = 0.0;
+  for (int row = 0; row < m; ++row) {
+    double a = 0.0;
+    for (int p = M->rowptr[row]; p < M->rowptr[row + 1]; ++p) a += M->vals[p] * z[M->colidx[p]];
+    r[row] = a;
+    rmax = std::max(rmax, std::fabs(a));
+  }
+  for (int row = 0; row < m; ++row)
+    for (int p = M->rowptr[row]; p < M->rowptr[row + 1]; ++p) t[M->colidx[p]] += M->vals[p] * r[row];
+  if (!(rmax > 0.0)) return false;
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return false;
+  DevBuf<double> dB, dG, dt, dv, dz, dw;
+  DevBuf<int> drp, dci, dinfo;
+  bool ok = false;
+  do {
+    const size_t nm = static_cast<size_t>(n) * m, nn = static_cast<size_t>(n) * n;
+    if (dB.ensure(nm) != hipSuccess || dG.ensure(nn) != hipSuccess || dinfo.ensure(1) != hipSuccess ||
+        drp.upload(M->rowptr.data(), M->rowptr.size(), nullptr) != hipSuccess ||
+        dci.upload(M->colidx.data(), std::max<size_t>(M->colidx.size(), 1), nullptr) != hipSuccess ||
+        dv.upload(M->vals.data(), std::max<size_t>(M->vals.size(), 1), nullptr) != hipSuccess ||
+        dt.upload(t.data(), t.size(), nullptr) != hipSuccess || dz.upload(z.data(), z.size(), nullptr) != hipSuccess ||
+        dw.ensure(std::max(d, 1)) != hipSuccess)
+      break;
+    if (hipMemset(dB.p, 0, sizeof(double) * nm) != hipSuccess ||
+        achr::launch_dense_st(m, n, drp.p, dci.p, dv.p, dB.p, nullptr) != hipSuccess)
+      break;
+    const double one = 1.0, zero = 0.0;
+    rocblas_int info = 1;
+    if (rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_none, n, m, &one, dB.p, n, &zero, dG.p, n) !=
+            rocblas_status_success ||
+        (d > 0 && rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_none, n, d, &sigma, dN, n, &one, dG.p, n) !=
+                      rocblas_status_success) ||
+        rocsolver_dpotrf(h, rocblas_fill_upper, n, dG.p, n, dinfo.p) != rocblas_status_success ||
+        hipMemcpy(&info, dinfo.p, sizeof info, hipMemcpyDeviceToHost) != hipSuccess || info != 0)
+      break;
+    // y = G^-1 S^T r = S+ r; S y must give back r
+    if (rocsolver_dpotrs(h, rocblas_fill_upper, n, 1, dG.p, n, dt.p, n) != rocblas_status_success ||
+        hipMemcpy(y.data(), dt.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      break;
+    double err = 0.0;
+    for (int row = 0; row < m; ++row) {
+      double a = 0.0;
+      for (int p = M->rowptr[row]; p < M->rowptr[row + 1]; ++p) a += M->vals[p] * y[M->colidx[p]];
+      err = std::max(err, std::fabs(a - r[row]));
+    }
+    // dz <- z - N (N^T z), the row-space part of z
+    const double minus_one = -1.0;
+    if (d > 0 && (rocblas_dgemv(h, rocblas_operation_transpose, n, d, &one, dN, n, dz.p, 1, &zero, dw.p, 1) !=
+                      rocblas_status_success ||
+                  rocblas_dgemv(h, rocblas_operation_none, n, d, &minus_one, dN, n, dw.p, 1, &one, dz.p, 1) !=
+                      rocblas_status_success))
+      break;
+    std::vector<double> zr(n);
+    if (hipMemcpy(zr.data(), dz.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) break;
+    double zmax = 0.0, dev = 0.0;
+    for (int i = 0; i < n; ++i) {
+      zmax = std::max(zmax, std::fabs(z[i]));
+      dev = std::max(dev, std::fabs(y[i] - zr[i]));
+    }
+    if (!(err <= 1e-6 * rmax) || !(dev <= 1e-7 * zmax)) break;
+    if (rocsolver_dpotrs(h, rocblas_fill_upper, n, m, dG.p, n, dB.p, n) != rocblas_status_success) break;
+    if (out.ensure(nm) != hipSuccess || achr::launch_to_f32(dB.p, out.p, nm, nullptr) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+      break;
+    ok = true;
+  } while (false);
+  rocblas_destroy_handle(h);
+  if (!ok) (void)hipGetLastError();
+  return ok;
+}
+
+static int build_pinv_f32(const achr_model* M, const double* dN, DevBuf<float>& out) {
   const int n = M->n, m = M->m;
   if (m == 0) {
     HIP_TRY(out.ensure(1));
     return ACHR_OK;
   }
+  if (pinv_chol(M, dN, out)) return ACHR_OK;
   // Columns of S (CSC): rows ascending within a column.
   std::vector<int> colptr(static_cast<size_t>(n) + 1, 0);
   for (int r = 0; r < m; ++r)
@@ -392,7 +495,7 @@ static int sampler_create(const achr_model* M, const double* warmup, int W, cons
   if (e != hipSuccess) return fail(ACHR_E_DEVICE, "sampler upload: %s", hipGetErrorString(e));
   if (o.proj_prec == ACHR_PROJ_RES_BF16X3 || o.proj_prec == ACHR_PROJ_RES_F32) {
     DevBuf<float> Sp32;  // S+, n x m column major (i contiguous)
-    const int rc = build_pinv_f32(M, Sp32);
+    const int rc = build_pinv_f32(M, S->Ncm.p, Sp32);
     if (rc != ACHR_OK) return rc;
     // B(k, i) = S+(i, k) for the rounds' X -= R S+^T, packed once
     const int gp = o.proj_prec == ACHR_PROJ_RES_F32 ? achr::kPrecF32 : achr::kPrecBF16x3;

@@ -148,6 +148,10 @@ hipError_t launch_add_cols_fix(double* X, const double* v, const double* lb, con
                                hipStream_t stream);
 // S+ = S^T G (n x m column major, FP32) from the CSC form of S (colptr n+1,
 // rowidx / vals per column) and the symmetric m x m FP64 matrix G.
+// B[i + r*n] = sum of the entries of S at (r, i): the dense S^T (n x m,
+// column major) from the CSR rows, B zeroed by the caller.
+hipError_t launch_dense_st(int m, int n, const int* rowptr, const int* colidx, const double* vals, double* B,
+                           hipStream_t stream);
 hipError_t launch_pinv_csc(int n, int m, const int* colptr, const int* rowidx, const double* vals, const double* G,
                            float* out, hipStream_t stream);
 // Elementwise FP64 -> FP32 copy of `count` values.

@@ -403,6 +403,16 @@ namespace {
 // S+[i + k*n] = sum over the entries (r, v) of column i of S of v * G[r + k*m],
 // accumulated in FP64 in the column's row order, rounded once. Threads run
 // over i, so stores are coalesced; the reads of G gather within one column.
+// One thread per row r of S adds the row's entries into column r of S^T.
+__global__ __launch_bounds__(256) void dense_st_kernel(int m, int n, const int* __restrict__ rowptr,
+                                                       const int* __restrict__ colidx, const double* __restrict__ vals,
+                                                       double* __restrict__ B) {
+  const int r = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+  if (r >= m) return;
+  double* col = B + static_cast<size_t>(r) * n;
+  for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) col[colidx[p]] += vals[p];
+}
+
 __global__ __launch_bounds__(256) void pinv_csc_kernel(int n, int m, const int* __restrict__ colptr,
                                                        const int* __restrict__ rowidx, const double* __restrict__ vals,
                                                        const double* __restrict__ G, float* __restrict__ out) {
@@ -454,6 +464,14 @@ hipError_t launch_add_cols_fix(double* X, const double* v, const double* lb, con
   if (total == 0) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>(std::min<size_t>((total + 255) / 256, 65536));
   hipLaunchKernelGGL(add_cols_fix_kernel, dim3(blocks), dim3(256), 0, stream, X, v, lb, ub, n, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_st(int m, int n, const int* rowptr, const int* colidx, const double* vals, double* B,
+                           hipStream_t stream) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dense_st_kernel, dim3(static_cast<unsigned>((m + 255) / 256)), dim3(256), 0, stream, m, n, rowptr,
+                     colidx, vals, B);
   return hipGetLastError();
 }
 
