@@ -379,57 +379,82 @@ static int free_basis(const achr_model* M, const double* dN, DevBuf<double>& Nf,
     *df_out = d;
     return ACHR_OK;
   }
-  // G = N_F^T N_F (d x d) on the host from the nf rows, eigenvectors on the GPU.
+  // The fixed rows N_F (nf x d) span, within the coordinates of N, the
+  // directions that move a fixed reaction; N_f = N Q2 with Q2 an orthonormal
+  // basis of their complement. The nf x nf Gram N_F N_F^T has the nonzero
+  // eigenvalues of N_F^T N_F: its eigenpairs at or above d * lambda_max *
+  // 2.2204e-16 give the range R = N_F^T U / sqrt(lambda) (d x r0), and the
+  // last d - r0 columns of the Householder Q of R are Q2 (rocSOLVER geqrf and
+  // orgqr), in place of an eigen-decomposition of the d x d N_F^T N_F.
   std::vector<double> NF(static_cast<size_t>(nf) * d);
   for (int r = 0; r < nf; ++r)
     for (int j = 0; j < d; ++j) NF[static_cast<size_t>(r) + static_cast<size_t>(j) * nf] = M->N[static_cast<size_t>(fixed[r]) + static_cast<size_t>(j) * n];
   rocblas_handle h = nullptr;
   if (rocblas_create_handle(&h) != rocblas_status_success) return fail(ACHR_E_SOLVER, "rocblas_create_handle");
-  DevBuf<double> dNF, dG, dD, dE, dZ;
+  DevBuf<double> dNF, dA, dD, dE, dU, dQ, dtau;
   DevBuf<int> dinfo;
   int rc = ACHR_OK;
-  std::vector<double> ev(d);
-  std::vector<double> Gh(static_cast<size_t>(d) * d);
+  std::vector<double> ev(nf), Ah(static_cast<size_t>(nf) * nf);
   const double one = 1.0, zero = 0.0;
   do {
-    if (dNF.upload(NF.data(), NF.size(), nullptr) != hipSuccess || dG.ensure(static_cast<size_t>(d) * d) != hipSuccess ||
-        dD.ensure(d) != hipSuccess || dE.ensure(d) != hipSuccess || dinfo.ensure(1) != hipSuccess) {
+    if (dNF.upload(NF.data(), NF.size(), nullptr) != hipSuccess || dA.ensure(static_cast<size_t>(nf) * nf) != hipSuccess ||
+        dD.ensure(nf) != hipSuccess || dE.ensure(nf) != hipSuccess || dinfo.ensure(1) != hipSuccess) {
       rc = fail(ACHR_E_NOMEM, "free basis buffers");
       break;
     }
-    if (rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, d, d, nf, &one, dNF.p, nf, dNF.p, nf,
-                      &zero, dG.p, d) != rocblas_status_success ||
-        rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, d, dG.p, d, dD.p, dE.p, dinfo.p) !=
-            rocblas_status_success) {
+    rocblas_int info = 0;
+    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, nf, nf, d, &one, dNF.p, nf, dNF.p, nf,
+                      &zero, dA.p, nf) != rocblas_status_success ||
+        rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, nf, dA.p, nf, dD.p, dE.p, dinfo.p) !=
+            rocblas_status_success ||
+        hipMemcpy(&info, dinfo.p, sizeof info, hipMemcpyDeviceToHost) != hipSuccess || info != 0) {
       rc = fail(ACHR_E_SOLVER, "free basis eigenproblem");
       break;
     }
-    if (hipMemcpy(ev.data(), dD.p, sizeof(double) * d, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(Gh.data(), dG.p, sizeof(double) * Gh.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpy(ev.data(), dD.p, sizeof(double) * nf, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(Ah.data(), dA.p, sizeof(double) * Ah.size(), hipMemcpyDeviceToHost) != hipSuccess) {
       rc = fail(ACHR_E_DEVICE, "free basis download");
       break;
     }
     double lmax = 0.0;
     for (double v : ev) lmax = std::max(lmax, std::fabs(v));
     const double tol = std::max(1.0, static_cast<double>(d)) * lmax * 2.2204e-16;
-    std::vector<int> keep;
-    for (int i = 0; i < d; ++i)
-      if (std::fabs(ev[i]) < tol) keep.push_back(i);
-    const int df = static_cast<int>(keep.size());
-    if (df == 0) {
+    // U / sqrt(lambda) for the range eigenpairs (nf x r0)
+    std::vector<double> Us;
+    int r0 = 0;
+    for (int k = 0; k < nf; ++k) {
+      if (!(std::fabs(ev[k]) >= tol) || !(ev[k] > 0.0)) continue;
+      const double sc = 1.0 / std::sqrt(ev[k]);
+      for (int i = 0; i < nf; ++i) Us.push_back(Ah[static_cast<size_t>(i) + static_cast<size_t>(k) * nf] * sc);
+      ++r0;
+    }
+    const int df = d - r0;
+    if (df <= 0) {
       rc = fail(ACHR_E_STATE, "the fixed reactions leave no free direction");
       break;
     }
-    std::vector<double> Zh(static_cast<size_t>(d) * df);
-    for (int j = 0; j < df; ++j)
-      std::copy(Gh.begin() + static_cast<size_t>(keep[j]) * d, Gh.begin() + static_cast<size_t>(keep[j] + 1) * d,
-                Zh.begin() + static_cast<size_t>(j) * d);
-    if (dZ.upload(Zh.data(), Zh.size(), nullptr) != hipSuccess || Nf.ensure(static_cast<size_t>(n) * df) != hipSuccess) {
+    if (r0 == 0) {
+      if (Nf.ensure(static_cast<size_t>(n) * d) != hipSuccess ||
+          hipMemcpy(Nf.p, dN, sizeof(double) * n * d, hipMemcpyDeviceToDevice) != hipSuccess) {
+        rc = fail(ACHR_E_NOMEM, "free basis");
+        break;
+      }
+      *df_out = d;
+      break;
+    }
+    // dQ (d x d): R = N_F^T (U / sqrt(lambda)) in its first r0 columns, then Q
+    if (dU.upload(Us.data(), Us.size(), nullptr) != hipSuccess || dQ.ensure(static_cast<size_t>(d) * d) != hipSuccess ||
+        dtau.ensure(r0) != hipSuccess || Nf.ensure(static_cast<size_t>(n) * df) != hipSuccess) {
       rc = fail(ACHR_E_NOMEM, "free basis");
       break;
     }
-    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, df, d, &one, dN, n, dZ.p, d, &zero, Nf.p,
-                      n) != rocblas_status_success || hipDeviceSynchronize() != hipSuccess) {
+    if (rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, d, r0, nf, &one, dNF.p, nf, dU.p, nf,
+                      &zero, dQ.p, d) != rocblas_status_success ||
+        rocsolver_dgeqrf(h, d, r0, dQ.p, d, dtau.p) != rocblas_status_success ||
+        rocsolver_dorgqr(h, d, d, r0, dQ.p, d, dtau.p) != rocblas_status_success ||
+        rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, df, d, &one, dN, n,
+                      dQ.p + static_cast<size_t>(r0) * d, d, &zero, Nf.p, n) != rocblas_status_success ||
+        hipDeviceSynchronize() != hipSuccess) {
       rc = fail(ACHR_E_SOLVER, "free basis product");
       break;
     }
@@ -438,11 +463,6 @@ static int free_basis(const achr_model* M, const double* dN, DevBuf<double>& Nf,
   rocblas_destroy_handle(h);
   return rc;
 }
-
-// Runs f(t, T) on T host threads (at most 16; the calling thread is one of
-// them, and it also takes the share of any thread that cannot be started).
-// Each f writes its own part of the outputs, so the results do not depend on
-// the thread count.
 
 // Cholesky-QR (twice) of the df x q FP64 matrix G (column major, host): G <- Q.
 // The products run on host threads, each element summed in the sequential
