@@ -22,7 +22,7 @@ struct DirNet {
   float *f = nullptr, *v = nullptr, *dv = nullptr, *df = nullptr;
   float* scratch = nullptr;
   size_t scratch_cap = 0;
-  void* blas = nullptr;     // rocblas_handle of the weight-gradient GEMMs (created on first use)
+  void* blas = nullptr;     // rocblas_handle of the FP32 training GEMMs (created on first use)
   float* cov = nullptr;     // O x O: V^T V, then the error V^T V / B - C
   float* target = nullptr;  // O x O target covariance C
   float target_norm2 = 1.0f;
