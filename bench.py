@@ -759,7 +759,7 @@ def compare_endpoints(m, A_pts, B_pts, P, ks_samples=4096):
 
 def gate(out, q, plain_qs):
     """The headline target's conditions, stated as booleans: per-reaction
-    mean and std within 1% of range of plain ACHR (the judge), drift <= 1e-9,
+    mean and std within 1% of range of plain ACHR (the reference sampler), drift <= 1e-9,
     and every sampler mixed (max split-R-hat < 1.01 and < 1.1)."""
     me = out.get("moment_err_vs_plain", {})
     rhats = [q["max_rhat"]] + [p["max_rhat"] for p in plain_qs]
@@ -773,9 +773,9 @@ def gate(out, q, plain_qs):
          "speedup_ge_10x_incl_training": bool(out.get("speedup_vs_plain_incl_training", 0) >= 10.0)}
     g["all"] = bool(g["mean_le_1pct_of_range"] and g["std_le_1pct_of_range"] and g["drift_le_1e-9"]
                     and g["rhat_lt_1.1"] and g["speedup_ge_10x"])
-    g["judge_mixed"] = bool(max(rhats) < 1.1)
+    g["reference_mixed"] = bool(max(rhats) < 1.1)
     g["note"] = ("The moment comparison is against plain ACHR, which is itself unmixed at this length when "
-                 "judge_mixed is false: min-ESS then measures the chains' transient motion from their start, and "
+                 "reference_mixed is false: min-ESS then measures the chains' transient motion from their start, and "
                  "moment differences compare two transients, not an error against the uniform law. No sampler "
                  "run here mixes on the 10,600-reaction model within a feasible run.")
     return g
